@@ -1,0 +1,211 @@
+/* rgs.h - C ABI of the MI355X Gaussian-splatting render/train path.
+ *
+ * Drop-in boundary for RustGS's rasterizer/trainer seams (reference paths are
+ * relative to the RustScan repository).  Every entry point names the
+ * reference interface it replaces.  All functions return RGS_OK (0) or a
+ * positive status; they never abort.  rgs_last_error() gives the message.
+ *
+ * Conventions
+ *  - One context = one HIP stream = one host thread at a time.  Entry points
+ *    are asynchronous on that stream unless stated otherwise.
+ *  - Array arguments are plain pointers; unless stated they may be host or
+ *    device memory (copies use unified addressing).  Arguments documented as
+ *    "device" must be device memory.
+ *  - Layouts at the boundary are the reference's
+ *    (training/engine/splats.rs:7-18):
+ *      transforms [N][10] = mean xyz, quaternion wxyz, log-scale xyz
+ *      sh         [N][K][3], K = (degree+1)^2, coefficient-major, RGB interleaved
+ *      raw_opac   [N]        opacity logits
+ *    Images are [H][W][4] RGBA float (forward/rasterize.rs:84-85), targets
+ *    [H][W][3] RGB float (engine/trainer.rs:265-268).
+ *  - Internally the splat store is SoA planes in HBM; the reference AoS layout
+ *    exists only at import/export.
+ */
+#ifndef RGS_H
+#define RGS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RGS_OK = 0,
+  RGS_ERR_INVALID = 1,     /* bad argument */
+  RGS_ERR_CAPACITY = 2,    /* a capacity was exceeded; retry after it grows */
+  RGS_ERR_HIP = 3,         /* HIP runtime error */
+  RGS_ERR_STATE = 4,       /* call out of order (e.g. backward without forward) */
+  RGS_ERR_UNSUPPORTED = 5  /* configuration not built (e.g. SH degree > 4) */
+};
+
+typedef struct rgs_ctx rgs_ctx;
+typedef struct rgs_splats rgs_splats;
+
+/* Camera of one view: core/camera.rs:30-37 (view_matrix, position) plus the
+ * intrinsics of rustscan-types/src/camera.rs:15-29, as packed into
+ * ProjectUniforms (forward/mod.rs:31-68). */
+typedef struct rgs_camera {
+  float viewmat[16]; /* world->camera, column-major (glam Mat4::to_cols_array) */
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+  float cam_pos[3]; /* camera centre in world space */
+} rgs_camera;
+
+/* AdamScaled (engine/optimizer.rs:7-24, 122-162) with the per-column
+ * learning-rate scaling tensors of trainer.rs:127-151 folded in:
+ * lr_transforms[10] = lr * transform scaling, lr_sh[25] = lr * SH scaling per
+ * coefficient (first K used), lr_opacity = lr * opacity scaling. */
+typedef struct rgs_adam_cfg {
+  float beta1, beta2, eps;
+  float lr_transforms[10];
+  float lr_sh[25];
+  float lr_opacity;
+} rgs_adam_cfg;
+
+enum { RGS_MODE_PER_VIEW = 0, RGS_MODE_BATCHED = 1 };
+enum { RGS_GRAPH_OFF = 0, RGS_GRAPH_ON = 1 };
+
+/* One training step over n_views views (WgpuTrainer::train_step,
+ * engine/trainer.rs:253-440, generalised to several views per step):
+ * forward, L1 loss with weight loss_weight per view (loss.rs:144-196,
+ * mean over H*W*3), backward summed over views, one Adam update. */
+typedef struct rgs_step_cfg {
+  int mode;          /* RGS_MODE_PER_VIEW: views one after another, grads
+                        accumulated; RGS_MODE_BATCHED: all views per launch */
+  int graph;         /* RGS_GRAPH_ON: capture the step as a HIP graph on first
+                        use and replay it while shapes and capacities hold */
+  float cov_blur;    /* ProjectUniforms::cov_blur (clamped at 0) */
+  float background[3];
+  float loss_weight; /* L1 weight per view */
+  int check_loss;    /* skip the update when the loss is not finite
+                        (trainer.rs:297-302) */
+  rgs_adam_cfg adam;
+} rgs_step_cfg;
+
+typedef struct rgs_raster_cfg {
+  uint32_t tile_width;  /* 8, 16 or 32 (reference: 16, forward/mod.rs:25) */
+  uint32_t batch_depth; /* LDS splat batch: 128, 256 or 512 (reference: 256) */
+  uint32_t persistent;  /* 1: persistent tile-queue launch, 0: one workgroup per tile */
+} rgs_raster_cfg;
+
+typedef struct rgs_counts {
+  int64_t num_visible;       /* V: Gaussians passing the projection cull */
+  int64_t num_intersections; /* M: (tile, Gaussian) pairs */
+  int64_t num_tiles;
+} rgs_counts;
+
+/* ---- context (replaces GsBackendBase/GsDevice, engine/backend.rs:9-18) ---- */
+int rgs_ctx_create(int device, rgs_ctx** out);
+void rgs_ctx_destroy(rgs_ctx* ctx);
+const char* rgs_last_error(const rgs_ctx* ctx);
+int rgs_sync(rgs_ctx* ctx);              /* blocks until the stream is idle */
+void* rgs_get_stream(rgs_ctx* ctx);      /* hipStream_t of the context */
+int rgs_set_raster_config(rgs_ctx* ctx, const rgs_raster_cfg* cfg);
+int rgs_get_raster_config(rgs_ctx* ctx, rgs_raster_cfg* cfg);
+
+/* ---- splat store (replaces DeviceSplats, engine/splats.rs:13-63) ---- */
+int rgs_splats_create(rgs_ctx* ctx, int64_t n, int sh_degree, rgs_splats** out);
+void rgs_splats_destroy(rgs_ctx* ctx, rgs_splats* s);
+int64_t rgs_splats_count(const rgs_splats* s);
+int rgs_splats_sh_degree(const rgs_splats* s);
+/* host_splats_to_device (splats.rs:26-63); resets the Adam state */
+int rgs_splats_import_aos(rgs_ctx* ctx, rgs_splats* s, const float* transforms, const float* sh,
+                          const float* raw_opac);
+/* device_splats_to_host (splats.rs:65-...); synchronous */
+int rgs_splats_export_aos(rgs_ctx* ctx, const rgs_splats* s, float* transforms, float* sh, float* raw_opac);
+/* Gradients of the last backward (or train step), reference layouts:
+ * v_transforms [N][10], v_sh [N][K][3], v_raw_opac [N]; screen_stats [N][7]
+ * (project_backwards.wgsl:33-36,106-108, summed over views); synchronous.
+ * Any pointer may be NULL. */
+int rgs_splats_export_grads(rgs_ctx* ctx, const rgs_splats* s, float* v_transforms, float* v_sh,
+                            float* v_raw_opac, float* screen_stats);
+/* visible[N]: 1.0 where some fragment of the last forward blended the
+ * Gaussian, OR over views (rasterize.wgsl:79); synchronous */
+int rgs_splats_export_visible(rgs_ctx* ctx, const rgs_splats* s, float* visible);
+/* AdamScaled::reset (optimizer.rs:62-66): zero moments and step counter */
+int rgs_adam_reset(rgs_ctx* ctx, rgs_splats* s);
+
+/* ---- forward: render_forward (forward/mod.rs:156-294) --------------------
+ * Projection + SH (ProjectionBackend, projection.rs:44-50; ProjectVisibleBackend,
+ * project_visible.rs:36-45), depth sort (SortingBackend, sorting.rs:8-61), tile
+ * binning (TileMappingBackend, tile_mapping.rs:82-95), compositing
+ * (RasterizeBackend, rasterize.rs:48-59) for n_views cameras in one batch.
+ * out_rgba: n_views device pointers (H*W*4 floats each) or NULL to keep the
+ * images internal.  Keeps the forward state for rgs_render_backward. */
+int rgs_render_forward(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, int n_views, const float* background,
+                       float cov_blur, float* const* out_rgba);
+
+/* ---- backward: RenderBackward::backward (backward/autodiff.rs:85-141) ----
+ * Rasterize backward (RasterizeBwdBackend, rasterize_bwd.rs:41-55) and
+ * projection backward (ProjectBwdBackend, project_bwd.rs:44-51) of the last
+ * rgs_render_forward.  v_out: n_views pointers to [H][W][4] image gradients.
+ * Gradients are summed over views into the splat store (see
+ * rgs_splats_export_grads). */
+int rgs_render_backward(rgs_ctx* ctx, rgs_splats* s, const float* const* v_out);
+
+/* ---- L1 loss (engine/loss.rs:144-196, trainer.rs:278) ---------------------
+ * pred [H][W][4], target [H][W][3] (device); v_out [H][W][4] (device, may be
+ * NULL); *loss_out (host) = weight * mean |pred_rgb - target|.  Synchronous
+ * when loss_out is non-NULL. */
+int rgs_l1_loss_grad(rgs_ctx* ctx, const float* pred, const float* target, uint32_t width, uint32_t height,
+                     float weight, float* v_out, float* loss_out);
+
+/* ---- Adam: AdamScaled::step_device_splats (optimizer.rs:80-120) -----------
+ * One update of all parameter groups from the splat store's gradients. */
+int rgs_adam_step(rgs_ctx* ctx, rgs_splats* s, const rgs_adam_cfg* cfg);
+
+/* ---- fused training step: WgpuTrainer::train_step (trainer.rs:253-440) ----
+ * targets: n_views device pointers to [H][W][3].  loss_out: n_views host
+ * floats or NULL (NULL keeps the step free of host synchronisation). */
+int rgs_train_step(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, const float* const* targets, int n_views,
+                   const rgs_step_cfg* cfg, float* loss_out);
+
+/* Picks tile width and LDS batch depth by timing rgs_train_step over the
+ * candidates {8,16,32} x {128,256,512} (persistent launch) on the given views;
+ * the choice is cached per (N, view sizes, SH degree) and made current. */
+int rgs_autotune(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, const float* const* targets, int n_views,
+                 const rgs_step_cfg* cfg, rgs_raster_cfg* chosen);
+
+/* Counts of the last forward per view (n_views entries); synchronous. */
+int rgs_get_counts(rgs_ctx* ctx, rgs_counts* out, int n_views);
+/* Number of steps whose update was skipped (capacity overflow or non-finite
+ * loss) since the context was created; synchronous. */
+int64_t rgs_skipped_steps(rgs_ctx* ctx);
+
+/* ---- GPU primitives (gpu_primitives/, device pointers) ---------------------
+ * RadixSortBackend::radix_sort_by_key_u32_primitive (radix_sort.rs:48-53):
+ * stable ascending sort of (key, value) pairs.
+ * PrefixSumBackend::prefix_sum_u32_primitive (prefix_sum.rs:41-44):
+ * inclusive scan. */
+int rgs_sort_pairs_u32(rgs_ctx* ctx, const uint32_t* keys, const uint32_t* values, uint32_t* out_keys,
+                       uint32_t* out_values, int64_t n);
+int rgs_inclusive_scan_u32(rgs_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n);
+
+/* ---- intermediate state of the last forward, for parity checks ------------
+ * Copied into host or device memory; synchronous.  `view` selects a view of
+ * the last batch; returns the element count through *count (call with
+ * dst = NULL to query). */
+enum {
+  RGS_DEBUG_DEPTH_ORDER = 0, /* uint32 gids of visible Gaussians, front to back */
+  RGS_DEBUG_ISECT_TILES = 1, /* uint32 tile ids (view-local) of sorted intersections */
+  RGS_DEBUG_ISECT_GIDS = 2,  /* uint32 gids of sorted intersections */
+  RGS_DEBUG_TILE_RANGES = 3, /* uint32 [2*T] (start, end) per tile, view-local offsets */
+  RGS_DEBUG_PROJECTED = 4,   /* float [N][9] x,y,a,b,c,r,g,b,opacity by gid (0 if culled) */
+  RGS_DEBUG_TILE_COUNTS = 5, /* uint32 [N] tiles per Gaussian */
+  RGS_DEBUG_IMAGE = 6,       /* float [H][W][4] */
+  RGS_DEBUG_RASTER_GRADS = 7 /* float [N][16] rasterizer accumulators of the last backward
+                                (v_xy, v_conic, v_rgb, v_opacity, v_refine, |v_xy|, coverage) */
+};
+int rgs_debug_get(rgs_ctx* ctx, int what, int view, void* dst, int64_t* count);
+/* flags bit 0: keep a copy of the rasterizer accumulators of each backward
+ * (RGS_DEBUG_RASTER_GRADS); costs one extra copy per backward. */
+int rgs_set_debug(rgs_ctx* ctx, int flags);
+
+/* Library version string. */
+const char* rgs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RGS_H */

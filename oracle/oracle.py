@@ -1,0 +1,309 @@
+"""ctypes wrapper of the oracle (oracle/_build/librgs_oracle.so).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py use this module, as the checker or the timed CPU baseline.
+The C++ it wraps (rgs_oracle.cpp) restates RustGS's WGSL pipeline; see the
+file:line citations there.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_int64, c_uint32, c_void_p
+
+import numpy as np
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "librgs_oracle.so")
+
+
+class OrcCamera(ctypes.Structure):
+    _fields_ = [
+        ("viewmat", c_float * 16),
+        ("fx", c_float),
+        ("fy", c_float),
+        ("cx", c_float),
+        ("cy", c_float),
+        ("width", c_uint32),
+        ("height", c_uint32),
+        ("cam_pos", c_float * 3),
+    ]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [
+        ("num_visible", c_int64),
+        ("num_intersections", c_int64),
+        ("k1_count_mismatch", c_int64),
+        ("depth_tie_runs", c_int64),
+        ("evaluated_fragments", c_int64),
+        ("blended_fragments", c_int64),
+        ("num_tiles", c_int64),
+    ]
+
+
+_LIB = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_forward.restype = c_void_p
+        L.orc_forward.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera), c_void_p, c_float,
+                                  c_int]
+        L.orc_free.argtypes = [c_void_p]
+        L.orc_stats.argtypes = [c_void_p, POINTER(OrcStats)]
+        for name in ["orc_get_image", "orc_get_visible", "orc_get_depth_order", "orc_get_depths", "orc_get_projected",
+                     "orc_get_counts", "orc_get_tile_offsets", "orc_render_bruteforce", "orc_crossing_splats"]:
+            getattr(L, name).argtypes = [c_void_p, c_void_p]
+        L.orc_get_isect.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.orc_backward.restype = c_int
+        L.orc_backward.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
+        L.orc_l1_loss_grad.restype = c_double
+        L.orc_l1_loss_grad.argtypes = [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p]
+        L.orc_adam_group.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+                                     c_float, c_float, ctypes.c_int32]
+        L.orc_powi.restype = c_float
+        L.orc_powi.argtypes = [c_float, ctypes.c_int32]
+        L.orc_forward_loss_f64.restype = c_double
+        L.orc_forward_loss_f64.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera),
+                                           c_void_p, c_double, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+        L.orc_projection_terms_f64.argtypes = [c_int64, c_void_p, c_void_p, POINTER(OrcCamera), c_double, c_void_p,
+                                               c_void_p]
+        L.orc_tiled_renderer.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         POINTER(OrcCamera), c_void_p, c_float, c_void_p, c_void_p]
+        L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
+        L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
+        L.orc_sh_color.argtypes = [c_int, c_void_p, c_void_p, c_void_p]
+        L.orc_sort_by_key.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.orc_inclusive_scan.argtypes = [c_int64, c_void_p, c_void_p]
+        L.orc_num_threads.restype = c_int
+        _LIB = L
+    return _LIB
+
+
+def camera(cam) -> OrcCamera:
+    c = OrcCamera()
+    for i in range(16):
+        c.viewmat[i] = float(cam.viewmat[i])
+    c.fx, c.fy, c.cx, c.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
+    c.width, c.height = int(cam.width), int(cam.height)
+    for i in range(3):
+        c.cam_pos[i] = float(cam.cam_pos[i])
+    return c
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Render:
+    """One FP32 forward of the oracle and its intermediates."""
+
+    def __init__(self, splats, cam, background=(0.0, 0.0, 0.0), cov_blur: float = 0.3, tile_width: int = 16):
+        L = lib()
+        self.n = splats.n
+        self.k = (splats.sh_degree + 1) ** 2
+        self.cam = cam
+        self._tf, self._sh, self._op = _f32(splats.transforms), _f32(splats.sh), _f32(splats.raw_opac)
+        bg = (c_float * 3)(*background)
+        self._c = camera(cam)
+        self.h = L.orc_forward(self.n, splats.sh_degree, self._tf.ctypes.data, self._sh.ctypes.data,
+                               self._op.ctypes.data, ctypes.byref(self._c), bg, float(cov_blur), int(tile_width))
+        if not self.h:
+            raise ValueError("oracle forward rejected its arguments")
+        st = OrcStats()
+        L.orc_stats(self.h, ctypes.byref(st))
+        self.stats = {f: getattr(st, f) for f, _ in OrcStats._fields_}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    def image(self) -> np.ndarray:
+        out = np.zeros((self.cam.height, self.cam.width, 4), np.float32)
+        lib().orc_get_image(self.h, out.ctypes.data)
+        return out
+
+    def visible(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        lib().orc_get_visible(self.h, out.ctypes.data)
+        return out
+
+    def depth_order(self) -> np.ndarray:
+        out = np.zeros(self.stats["num_visible"], np.uint32)
+        lib().orc_get_depth_order(self.h, out.ctypes.data)
+        return out
+
+    def depths(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        lib().orc_get_depths(self.h, out.ctypes.data)
+        return out
+
+    def projected(self) -> np.ndarray:
+        """[V,9] in depth order: x, y, a, b, c, r, g, b, opacity."""
+        out = np.zeros((self.stats["num_visible"], 9), np.float32)
+        lib().orc_get_projected(self.h, out.ctypes.data)
+        return out
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint32)
+        lib().orc_get_counts(self.h, out.ctypes.data)
+        return out
+
+    def isect(self):
+        m = self.stats["num_intersections"]
+        tiles = np.zeros(m, np.uint32)
+        gids = np.zeros(m, np.uint32)
+        lib().orc_get_isect(self.h, tiles.ctypes.data, gids.ctypes.data)
+        return tiles, gids
+
+    def tile_offsets(self) -> np.ndarray:
+        out = np.zeros(2 * self.stats["num_tiles"], np.uint32)
+        lib().orc_get_tile_offsets(self.h, out.ctypes.data)
+        return out
+
+    def bruteforce(self) -> np.ndarray:
+        out = np.zeros((self.cam.height, self.cam.width, 4), np.float32)
+        lib().orc_render_bruteforce(self.h, out.ctypes.data)
+        return out
+
+    def crossing(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint8)
+        lib().orc_crossing_splats(self.h, out.ctypes.data)
+        return out
+
+    def backward(self, v_out: np.ndarray, mode: int = 0):
+        vt = np.zeros((self.n, 10), np.float32)
+        vs = np.zeros((self.n, self.k, 3), np.float32)
+        vo = np.zeros(self.n, np.float32)
+        st = np.zeros((self.n, 7), np.float32)
+        V = self.stats["num_visible"]
+        vsp = np.zeros((V, 10), np.float32)
+        ssp = np.zeros((V, 5), np.float32)
+        v_out = _f32(v_out)
+        rc = lib().orc_backward(self.h, v_out.ctypes.data, mode, vt.ctypes.data, vs.ctypes.data, vo.ctypes.data,
+                                st.ctypes.data, vsp.ctypes.data, ssp.ctypes.data)
+        if rc:
+            raise ValueError("oracle backward failed")
+        return dict(v_transforms=vt, v_sh=vs, v_raw_opac=vo, screen_stats=st, v_splats=vsp, screen_splats=ssp)
+
+
+def l1_loss_grad(pred: np.ndarray, target: np.ndarray, weight: float = 1.0):
+    pred, target = _f32(pred), _f32(target)
+    h, w = pred.shape[:2]
+    v = np.zeros((h, w, 4), np.float32)
+    loss = lib().orc_l1_loss_grad(pred.ctypes.data, target.ctypes.data, w, h, float(weight), v.ctypes.data)
+    return loss, v
+
+
+def adam_group(param, grad, m, v, lr_row, beta1, beta2, eps, step):
+    """In-place Adam on float32 arrays of shape [rows, row_len]."""
+    for a in (param, grad, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    rows, row_len = param.shape
+    lr = _f32(lr_row)
+    lib().orc_adam_group(rows, row_len, param.ctypes.data, grad.ctypes.data, m.ctypes.data, v.ctypes.data,
+                         lr.ctypes.data, float(beta1), float(beta2), float(eps), int(step))
+
+
+def powi(base: float, e: int) -> float:
+    return lib().orc_powi(float(base), int(e))
+
+
+def forward_loss_f64(splats64, sh_degree, cam, target, cov_blur=0.3, tile_width=16, flags=0, frozen_viewdir=None,
+                     frozen_comp=None, background=(0.0, 0.0, 0.0)):
+    """FP64 forward + L1 loss (mean over 3HW) for finite differences."""
+    tf, sh, op = [np.ascontiguousarray(a, np.float64) for a in splats64]
+    n = op.shape[0]
+    c = camera(cam)
+    bg = np.ascontiguousarray(background, np.float64)
+    fv = np.ascontiguousarray(frozen_viewdir, np.float64) if frozen_viewdir is not None else None
+    fc = np.ascontiguousarray(frozen_comp, np.float64) if frozen_comp is not None else None
+    tg = _f32(target)
+    return lib().orc_forward_loss_f64(n, sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data, ctypes.byref(c),
+                                      bg.ctypes.data, float(cov_blur), int(tile_width), tg.ctypes.data, int(flags),
+                                      fv.ctypes.data if fv is not None else None,
+                                      fc.ctypes.data if fc is not None else None, None)
+
+
+def projection_terms_f64(tf64, op64, cam, cov_blur=0.3):
+    n = op64.shape[0]
+    vd = np.zeros((n, 3), np.float64)
+    comp = np.zeros(n, np.float64)
+    c = camera(cam)
+    tf64 = np.ascontiguousarray(tf64, np.float64)
+    op64 = np.ascontiguousarray(op64, np.float64)
+    lib().orc_projection_terms_f64(n, tf64.ctypes.data, op64.ctypes.data, ctypes.byref(c), float(cov_blur),
+                                   vd.ctypes.data, comp.ctypes.data)
+    return vd, comp
+
+
+def tiled_renderer(splats, cam, background=(0.0, 0.0, 0.0), cov_blur=0.3):
+    """The reference's CPU renderer (render/tiled_renderer.rs), single-threaded."""
+    tf = _f32(splats.transforms)
+    pos = np.ascontiguousarray(tf[:, 0:3])
+    rot = np.ascontiguousarray(tf[:, 3:7])
+    ls = np.ascontiguousarray(tf[:, 7:10])
+    op = _f32(splats.raw_opac)
+    sh = _f32(splats.sh)
+    c = camera(cam)
+    color = np.zeros((cam.height, cam.width, 3), np.float32)
+    depth = np.zeros((cam.height, cam.width), np.float32)
+    bg = (c_float * 3)(*background)
+    lib().orc_tiled_renderer(splats.n, splats.sh_degree, pos.ctypes.data, ls.ctypes.data, rot.ctypes.data,
+                             op.ctypes.data, sh.ctypes.data, ctypes.byref(c), bg, float(cov_blur), color.ctypes.data,
+                             depth.ctypes.data)
+    return color, depth
+
+
+def math_exp(x: np.ndarray) -> np.ndarray:
+    x = _f32(x)
+    y = np.zeros_like(x)
+    lib().orc_math_exp(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def math_log(x: np.ndarray) -> np.ndarray:
+    x = _f32(x)
+    y = np.zeros_like(x)
+    lib().orc_math_log(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def sh_color(degree: int, coeffs: np.ndarray, viewdir) -> np.ndarray:
+    co = _f32(coeffs)
+    vd = _f32(viewdir)
+    out = np.zeros(3, np.float32)
+    lib().orc_sh_color(degree, co.ctypes.data, vd.ctypes.data, out.ctypes.data)
+    return out
+
+
+def sort_by_key(keys, values):
+    k = np.ascontiguousarray(keys, np.uint32)
+    v = np.ascontiguousarray(values, np.uint32)
+    ok = np.zeros_like(k)
+    ov = np.zeros_like(v)
+    lib().orc_sort_by_key(k.size, k.ctypes.data, v.ctypes.data, ok.ctypes.data, ov.ctypes.data)
+    return ok, ov
+
+
+def inclusive_scan(x):
+    a = np.ascontiguousarray(x, np.uint32)
+    out = np.zeros_like(a)
+    lib().orc_inclusive_scan(a.size, a.ctypes.data, out.ctypes.data)
+    return out
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,136 @@
+/* Oracle for the Gaussian-splatting render/train path: a CPU restatement of
+ * RustGS's WGSL pipeline (the .wgsl files under RustGS/src/training/shaders/ and the Rust glue
+ * in RustGS/src/training/{forward,backward,engine}).
+ *
+ * THIS IS TEST INFRASTRUCTURE.  Only the test suite, the smoke check and the
+ * CPU-baseline leg of the benchmark may load it, and only as the checker (or
+ * the timed CPU baseline).  The product library never links or calls it.
+ *
+ * How each function is pinned (golden vectors, closed forms, brute force,
+ * finite differences) is stated in the docstrings of tests/test_oracle_*.py.
+ */
+#ifndef RGS_ORACLE_H
+#define RGS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as rgs_camera in include/rgs.h (kept separate so the oracle does
+ * not depend on the product header). */
+typedef struct orc_camera {
+  float viewmat[16]; /* world->camera, column-major (glam Mat4 cols) */
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+  float cam_pos[3]; /* camera centre in world space */
+} orc_camera;
+
+typedef struct orc_render orc_render;
+
+typedef struct orc_fwd_stats {
+  int64_t num_visible;        /* V: Gaussians passing the projection cull */
+  int64_t num_intersections;  /* M: (tile, Gaussian) pairs emitted */
+  int64_t k1_count_mismatch;  /* Gaussians whose K1 count != K5 emission */
+  int64_t depth_tie_runs;     /* runs of equal depth bits among visible */
+  int64_t evaluated_fragments;/* (pixel, splat) pairs evaluated by K7 */
+  int64_t blended_fragments;  /* (pixel, splat) pairs accepted by K7 */
+  int64_t num_tiles;
+} orc_fwd_stats;
+
+/* ---- forward (FP32, reference semantics, canonical tie rule) ---------- */
+orc_render* orc_forward(int64_t n, int sh_degree, const float* transforms,
+                        const float* sh, const float* raw_opac,
+                        const orc_camera* cam, const float* background,
+                        float cov_blur, int tile_width);
+void orc_free(orc_render* r);
+void orc_stats(const orc_render* r, orc_fwd_stats* out);
+void orc_get_image(const orc_render* r, float* out_rgba);   /* H*W*4 */
+void orc_get_visible(const orc_render* r, float* out);      /* N */
+void orc_get_depth_order(const orc_render* r, uint32_t* gids); /* V */
+void orc_get_depths(const orc_render* r, float* depth_by_gid); /* N, 0 if culled */
+void orc_get_projected(const orc_render* r, float* out);    /* V*9, compact order */
+void orc_get_counts(const orc_render* r, uint32_t* counts_by_gid); /* N */
+/* sorted intersections: tile ids and GLOBAL gids, M each */
+void orc_get_isect(const orc_render* r, uint32_t* tile_ids, uint32_t* gids);
+void orc_get_tile_offsets(const orc_render* r, uint32_t* offsets); /* 2*T */
+
+/* ---- backward ------------------------------------------------------------
+ * mode 0: reference semantics (rasterize_backwards.wgsl + project_backwards.wgsl)
+ * mode 1: exact gradient of orc_forward's image (crossing splat included,
+ *         forward sigma rounding, SH view-direction and blur-compensation
+ *         derivatives added).
+ * Outputs are overwritten (not accumulated); any pointer may be NULL. */
+int orc_backward(orc_render* r, const float* v_out_rgba, int mode,
+                 float* v_transforms, float* v_sh, float* v_raw_opac,
+                 float* screen_stats, float* v_splats_compact,
+                 float* screen_splats_compact);
+
+/* ---- L1 loss (engine/loss.rs reconstruction_residual_loss, plain L1) ---- */
+double orc_l1_loss_grad(const float* pred_rgba, const float* target_rgb,
+                        int64_t width, int64_t height, float weight,
+                        float* v_out_rgba);
+
+/* ---- Adam (engine/optimizer.rs step_tensor) ------------------------------
+ * One group: param/grad/m/v of n_rows*row_len floats; lr_row[row_len] is the
+ * per-column learning rate (lr * scaling); step is the group's step counter
+ * AFTER increment.  beta1/beta2/eps as in AdamScaledConfig. */
+void orc_adam_group(int64_t n_rows, int64_t row_len, float* param,
+                    const float* grad, float* m, float* v, const float* lr_row,
+                    float beta1, float beta2, float eps, int32_t step);
+float orc_powi(float base, int32_t exp);
+
+/* ---- FP64 forward for finite differences ---------------------------------
+ * Renders with double arithmetic and returns the L1 loss against target_rgb
+ * (weight 1/(3HW)), optionally writing the image.
+ * flags bit0: freeze the SH view direction to frozen_viewdir[N*3]
+ * flags bit1: freeze the blur compensation to frozen_comp[N]
+ * flags bit2: reference-backward termination (stop before the splat that
+ *             takes T to <= 1e-4 instead of after it)
+ * Tile lists are rebuilt from the double projection. */
+double orc_forward_loss_f64(int64_t n, int sh_degree, const double* transforms,
+                            const double* sh, const double* raw_opac,
+                            const orc_camera* cam, const double* background,
+                            double cov_blur, int tile_width, const float* target_rgb,
+                            int flags, const double* frozen_viewdir,
+                            const double* frozen_comp, double* out_rgba);
+/* The per-Gaussian view direction and compensation of the FP64 projection,
+ * for building the frozen surrogate (NaN rows for culled Gaussians). */
+void orc_projection_terms_f64(int64_t n, const double* transforms,
+                              const double* raw_opac, const orc_camera* cam,
+                              double cov_blur, double* viewdir, double* comp);
+/* Global ids of Gaussians that are the terminating ("crossing") splat of at
+ * least one pixel in the FP32 forward: 1/0 per gid. */
+void orc_crossing_splats(const orc_render* r, uint8_t* flags);
+
+/* ---- brute-force renderer: no tiles, every Gaussian tested at every pixel
+ * in depth order, same per-pixel arithmetic as K7.  Only valid where the tile
+ * test is exact (used on small scenes to check the binning). */
+void orc_render_bruteforce(const orc_render* r, float* out_rgba);
+
+/* ---- the reference's CPU renderer (render/tiled_renderer.rs), restated ----
+ * positions/log_scales [N*3], rotations wxyz [N*4], opacity_logits [N],
+ * sh [N*K*3].  Writes color[H*W*3], depth[H*W]. Single-threaded. */
+void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions,
+                        const float* log_scales, const float* rotations,
+                        const float* opacity_logits, const float* sh,
+                        const orc_camera* cam, const float* background,
+                        float cov_blur, float* color, float* depth);
+
+/* ---- scalar math exported for pinning tests ---- */
+void orc_math_exp(const float* x, float* y, int64_t n);
+void orc_math_log(const float* x, float* y, int64_t n);
+/* SH colour (project_visible.wgsl:44-127): coeffs [K*3], viewdir[3] -> rgb[3] */
+void orc_sh_color(int degree, const float* coeffs, const float* viewdir, float* rgb);
+/* sort contract: stable ascending by key, ties by input position */
+void orc_sort_by_key(int64_t n, const uint32_t* keys, const uint32_t* values,
+                     uint32_t* out_keys, uint32_t* out_values);
+void orc_inclusive_scan(int64_t n, const uint32_t* in, uint32_t* out);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

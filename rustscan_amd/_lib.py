@@ -1,0 +1,147 @@
+"""ctypes binding of librgs_hip.so (the C ABI declared in include/rgs.h).
+
+The library is the only implementation of the render/train path: there is no
+CPU fallback.  Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_float, c_int, c_int64, c_uint32, c_void_p
+
+from .build import LIB_PATH
+
+RGS_OK = 0
+RGS_ERR_INVALID = 1
+RGS_ERR_CAPACITY = 2
+RGS_ERR_HIP = 3
+RGS_ERR_STATE = 4
+RGS_ERR_UNSUPPORTED = 5
+
+RGS_MODE_PER_VIEW = 0
+RGS_MODE_BATCHED = 1
+RGS_GRAPH_OFF = 0
+RGS_GRAPH_ON = 1
+
+RGS_DEBUG_DEPTH_ORDER = 0
+RGS_DEBUG_ISECT_TILES = 1
+RGS_DEBUG_ISECT_GIDS = 2
+RGS_DEBUG_TILE_RANGES = 3
+RGS_DEBUG_PROJECTED = 4
+RGS_DEBUG_TILE_COUNTS = 5
+RGS_DEBUG_IMAGE = 6
+RGS_DEBUG_RASTER_GRADS = 7
+
+
+class RgsCamera(ctypes.Structure):
+    _fields_ = [
+        ("viewmat", c_float * 16),
+        ("fx", c_float),
+        ("fy", c_float),
+        ("cx", c_float),
+        ("cy", c_float),
+        ("width", c_uint32),
+        ("height", c_uint32),
+        ("cam_pos", c_float * 3),
+    ]
+
+
+class RgsAdamCfg(ctypes.Structure):
+    _fields_ = [
+        ("beta1", c_float),
+        ("beta2", c_float),
+        ("eps", c_float),
+        ("lr_transforms", c_float * 10),
+        ("lr_sh", c_float * 25),
+        ("lr_opacity", c_float),
+    ]
+
+
+class RgsStepCfg(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int),
+        ("graph", c_int),
+        ("cov_blur", c_float),
+        ("background", c_float * 3),
+        ("loss_weight", c_float),
+        ("check_loss", c_int),
+        ("adam", RgsAdamCfg),
+    ]
+
+
+class RgsRasterCfg(ctypes.Structure):
+    _fields_ = [("tile_width", c_uint32), ("batch_depth", c_uint32), ("persistent", c_uint32)]
+
+
+class RgsCounts(ctypes.Structure):
+    _fields_ = [("num_visible", c_int64), ("num_intersections", c_int64), ("num_tiles", c_int64)]
+
+
+class RgsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"rgs error {code}: {message}")
+        self.code = code
+
+
+_LIB = None
+
+_SIGNATURES = {
+    "rgs_ctx_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "rgs_ctx_destroy": (None, [c_void_p]),
+    "rgs_last_error": (ctypes.c_char_p, [c_void_p]),
+    "rgs_sync": (c_int, [c_void_p]),
+    "rgs_get_stream": (c_void_p, [c_void_p]),
+    "rgs_set_raster_config": (c_int, [c_void_p, POINTER(RgsRasterCfg)]),
+    "rgs_get_raster_config": (c_int, [c_void_p, POINTER(RgsRasterCfg)]),
+    "rgs_splats_create": (c_int, [c_void_p, c_int64, c_int, POINTER(c_void_p)]),
+    "rgs_splats_destroy": (None, [c_void_p, c_void_p]),
+    "rgs_splats_count": (c_int64, [c_void_p]),
+    "rgs_splats_sh_degree": (c_int, [c_void_p]),
+    "rgs_splats_import_aos": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rgs_splats_export_aos": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rgs_splats_export_grads": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rgs_splats_export_visible": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rgs_adam_reset": (c_int, [c_void_p, c_void_p]),
+    "rgs_render_forward": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
+                                   POINTER(c_void_p)]),
+    "rgs_render_backward": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
+    "rgs_l1_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, c_void_p,
+                                 POINTER(c_float)]),
+    "rgs_adam_step": (c_int, [c_void_p, c_void_p, POINTER(RgsAdamCfg)]),
+    "rgs_train_step": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_void_p), c_int,
+                               POINTER(RgsStepCfg), POINTER(c_float)]),
+    "rgs_autotune": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_void_p), c_int, POINTER(RgsStepCfg),
+                             POINTER(RgsRasterCfg)]),
+    "rgs_get_counts": (c_int, [c_void_p, POINTER(RgsCounts), c_int]),
+    "rgs_skipped_steps": (c_int64, [c_void_p]),
+    "rgs_sort_pairs_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
+    "rgs_inclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "rgs_debug_get": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(c_int64)]),
+    "rgs_set_debug": (c_int, [c_void_p, c_int]),
+    "rgs_version": (ctypes.c_char_p, []),
+}
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load librgs_hip.so from the package directory (raises if missing)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(ctx, code: int) -> None:
+    if code != RGS_OK:
+        msg = lib().rgs_last_error(ctx).decode() if ctx else ""
+        raise RgsError(code, msg)

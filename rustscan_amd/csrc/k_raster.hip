@@ -1,0 +1,348 @@
+// Per-tile alpha compositing, forward (rasterize.wgsl:21-99, K7) and backward
+// (rasterize_backwards.wgsl:24-169, K8).
+//
+// One workgroup composites one TWxTW tile, one thread per pixel.  The tile's
+// depth-sorted splats are staged through LDS in batches of BATCH.  Work items
+// are (view, tile) pairs numbered by global tile id: either one workgroup per
+// tile (grid launch) or a persistent grid pulling tiles from an atomic counter,
+// which is how uneven views share one launch.
+//
+// Forward changes versus the reference that do not change any result:
+//  * a workgroup stops loading batches once every pixel is done
+//    (__syncthreads_count), the reference keeps looping over its range;
+//  * exp is skipped when sigma > log(255*opacity) + 1e-3: there alpha < 1/255
+//    whatever exp returns, so the splat is rejected either way;
+//  * visibility flags are gathered per batch in LDS and written once per splat.
+// sigma is evaluated as ((a/2*dx)*dx + (b*dx)*dy) + (c/2*dy)*dy, which equals
+// K7's 0.5*(a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit (scaling by powers of
+// two commutes with rounding); the backward uses K8's own grouping.
+//
+// Backward: per splat the 64 lanes of a wave reduce their contributions
+// (12 values + a ballot count), lane 0 adds them into an LDS accumulator for
+// the batch, and after the batch one thread per splat adds the totals into the
+// per-instance accumulator with global atomics (one set per tile and splat
+// instead of one per pixel).
+#include "rgs_device.h"
+
+namespace rgs {
+
+
+
+constexpr float kSkipMargin = 1e-3f;
+
+__device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t* s_item, uint32_t& iter, uint32_t* item) {
+  if (A.queue) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(A.queue, 1u);
+    __syncthreads();
+    *item = *s_item;
+    __syncthreads();
+  } else {
+    if (iter++) return false;
+    *item = blockIdx.x;
+  }
+  return *item < A.tile_count;
+}
+
+__device__ __forceinline__ int view_of(const RasterArgs& A, uint32_t tile) {
+  int v = A.v0;
+  for (int i = A.v0 + 1; i < A.v0 + A.nv; ++i)
+    if (tile >= A.views[i].tile_base) v = i;
+  return v;
+}
+
+template <int TW, int BATCH>
+__global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
+  constexpr int NT = TW * TW;
+  __shared__ float4 s0[BATCH];  // x, y, a/2, b
+  __shared__ float4 s1[BATCH];  // c/2, opacity, max(r,0), max(g,0)
+  __shared__ float2 s2[BATCH];  // max(b,0), exp-skip bound
+  __shared__ uint32_t sg[BATCH];
+  __shared__ uint32_t sv[BATCH];
+  __shared__ uint32_t s_item;
+  const int tid = threadIdx.x;
+  uint32_t iter = 0, item;
+  while (next_item(A, &s_item, iter, &item)) {
+    const uint32_t tile = A.tile_first + item;
+    const int v = view_of(A, tile);
+    const ViewDev& V = A.views[v];
+    const uint32_t local = tile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t px = tx * TW + (tid % TW), py = ty * TW + (tid / TW);
+    const bool inside = px < V.W && py < V.H;
+    const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
+    const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    bool done = !inside;
+    for (uint32_t b = rs; b < re; b += BATCH) {
+      if (__syncthreads_count(!done) == 0) break;
+      const uint32_t cnt = min((uint32_t)BATCH, re - b);
+      for (uint32_t j = tid; j < cnt; j += NT) {
+        const uint32_t inst = A.isect_vals[b + j];
+        const ProjSplat q = A.proj[inst];
+        s0[j] = q.p0;
+        s1[j] = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
+        s2[j] = make_float2(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin);
+        sg[j] = inst - (uint32_t)v * A.n;
+        sv[j] = 0u;
+      }
+      __syncthreads();
+      if (!done) {
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const float4 a0 = s0[j];
+          const float dx = a0.x - pcx, dy = a0.y - pcy;
+          const float4 a1 = s1[j];
+          const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
+          const float2 a2 = s2[j];
+          if (sigma > a2.y) continue;
+          const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp(-sigma));
+          if (sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0)) {
+            const float vis = alpha * T;
+            cr = cr + a1.z * vis;
+            cg = cg + a1.w * vis;
+            cb = cb + a2.x * vis;
+            T = T * (1.0f - alpha);
+            sv[j] = 1u;
+            if (T < RGS_F(1e-4)) {
+              done = true;
+              break;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (A.visible)
+        for (uint32_t j = tid; j < cnt; j += NT)
+          if (sv[j]) A.visible[sg[j]] = 1.0f;
+    }
+    float l1 = 0.0f;
+    if (inside) {
+      const size_t pix = (size_t)py * V.W + px;
+      const float o0 = cr + T * V.bg[0], o1 = cg + T * V.bg[1], o2 = cb + T * V.bg[2];
+      reinterpret_cast<float4*>(V.img)[pix] = make_float4(o0, o1, o2, 1.0f - T);
+      if (A.loss && V.target) {
+        const float* t = V.target + pix * 3;
+        l1 = (fabsf(o0 - t[0]) + fabsf(o1 - t[1])) + fabsf(o2 - t[2]);
+      }
+    }
+    if (A.loss) {
+      const float s = wave_sum(l1);
+      if ((tid & 63) == 0 && s != 0.0f) atomicAdd(&A.loss[v], s);
+    }
+  }
+}
+
+template <int TW, int BATCH, bool REFINE>
+__global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
+  constexpr int NT = TW * TW;
+  constexpr int NA = 13;  // accumulated values per splat
+  __shared__ float4 s0[BATCH];  // x, y, a/2, b
+  __shared__ float4 s1[BATCH];  // c/2, opacity, r, g (unclamped)
+  __shared__ float2 s2[BATCH];  // b (unclamped), exp-skip bound
+  __shared__ uint32_t si[BATCH];
+  __shared__ float sacc[BATCH * NA];
+  __shared__ uint32_t s_item;
+  const int tid = threadIdx.x;
+  uint32_t iter = 0, item;
+  while (next_item(A, &s_item, iter, &item)) {
+    const uint32_t tile = A.tile_first + item;
+    const int v = view_of(A, tile);
+    const ViewDev& V = A.views[v];
+    const uint32_t local = tile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t px = tx * TW + (tid % TW), py = ty * TW + (tid / TW);
+    const bool inside = px < V.W && py < V.H;
+    const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
+    const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
+    // rasterize_backwards.wgsl:46-60
+    float vo[4] = {0.f, 0.f, 0.f, 0.f}, rgbf[4] = {0.f, 0.f, 0.f, 0.f};
+    if (inside) {
+      const size_t pix = (size_t)py * V.W + px;
+      const float4 fc = reinterpret_cast<const float4*>(V.img)[pix];
+      const float Tf = 1.0f - fc.w;
+      rgbf[0] = fc.x - Tf * V.bg[0];
+      rgbf[1] = fc.y - Tf * V.bg[1];
+      rgbf[2] = fc.z - Tf * V.bg[2];
+      rgbf[3] = fc.w;
+      float g[4];
+      if (V.vout) {
+        const float4 go = reinterpret_cast<const float4*>(V.vout)[pix];
+        g[0] = go.x; g[1] = go.y; g[2] = go.z; g[3] = go.w;
+      } else {
+        // L1 gradient (engine/loss.rs:144-196): sign(pred - target) * weight / (3HW); alpha gets none
+        const float* t = V.target + pix * 3;
+        const float d[3] = {fc.x - t[0], fc.y - t[1], fc.z - t[2]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] = d[c] > 0.0f ? V.loss_scale : (d[c] < 0.0f ? -V.loss_scale : 0.0f);
+        g[3] = 0.0f;
+      }
+      vo[0] = g[0]; vo[1] = g[1]; vo[2] = g[2];
+      vo[3] = (g[3] - ((V.bg[0] * g[0] + V.bg[1] * g[1]) + V.bg[2] * g[2])) * Tf;
+    }
+    float T = 1.0f;
+    float po[3] = {0.f, 0.f, 0.f};
+    bool done = !inside;
+    const float Wf = (float)V.W, Hf = (float)V.H;
+    for (uint32_t b = rs; b < re; b += BATCH) {
+      if (__syncthreads_count(!done) == 0) break;
+      const uint32_t cnt = min((uint32_t)BATCH, re - b);
+      for (uint32_t j = tid; j < cnt; j += NT) {
+        const uint32_t inst = A.isect_vals[b + j];
+        const ProjSplat q = A.proj[inst];
+        s0[j] = q.p0;
+        s1[j] = q.p1;
+        s2[j] = make_float2(q.p2.x, q.p2.y + kSkipMargin);
+        si[j] = inst;
+      }
+      for (uint32_t k = tid; k < cnt * NA; k += NT) sacc[k] = 0.0f;
+      __syncthreads();
+      for (uint32_t j = 0; j < cnt; ++j) {
+        if (__all(done)) break;
+        float val[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) val[k] = 0.0f;
+        bool contrib = false;
+        if (!done) {
+          const float4 a0 = s0[j];
+          const float4 a1 = s1[j];
+          const float2 a2 = s2[j];
+          const float dx = a0.x - pcx, dy = a0.y - pcy;
+          const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
+          if (!(sigma > a2.y)) {
+            const float gauss = rgs_exp(-sigma);
+            const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
+            if (!(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0))) {
+              const float nT = T * (1.0f - alpha);
+              if (nT <= RGS_F(1e-4)) {
+                done = true;
+              } else {
+                contrib = true;
+                const float col[3] = {a1.z, a1.w, a2.x};
+                const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
+                const float vis = alpha * T;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
+                  po[c] = po[c] + vis * cl[c];
+                }
+                const float ra = 1.0f / (1.0f - alpha);
+                float tv[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
+                const float v_alpha = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
+                const float v_sigma = -alpha * v_alpha;
+                const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
+                const float vx = v_sigma * (ca * dx + cb * dy);
+                const float vy = v_sigma * (cb * dx + cc * dy);
+                if (a1.y * gauss <= RGS_F(0.999)) {
+                  val[0] = vx;
+                  val[1] = vy;
+                  val[2] = 0.5f * v_sigma * dx * dx;
+                  val[3] = v_sigma * dx * dy;
+                  val[4] = 0.5f * v_sigma * dy * dy;
+                  val[8] = v_alpha * gauss;
+                  if (REFINE) {
+                    const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
+                    const float lx = vx * Wf, ly = vy * Hf;
+                    val[9] = rgs_sqrt(lx * lx + ly * ly) / fa;
+                  }
+                  val[10] = fabsf(vx);
+                  val[11] = fabsf(vy);
+                }
+                T = nT;
+              }
+            }
+          }
+        }
+        const uint64_t m = __ballot(contrib);
+        if (m == 0) continue;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          if (k == 9 && !REFINE) continue;
+          val[k] = wave_sum(val[k]);
+        }
+        if ((tid & 63) == 0) {
+          float* dst = sacc + j * NA;
+#pragma unroll
+          for (int k = 0; k < 12; ++k)
+            if (val[k] != 0.0f) atomicAdd(dst + k, val[k]);
+          atomicAdd(dst + 12, (float)__popcll(m));
+        }
+      }
+      __syncthreads();
+      for (uint32_t j = tid; j < cnt; j += NT) {
+        const float* src = sacc + j * NA;
+        if (src[12] == 0.0f) continue;
+        float* dst = A.acc + (size_t)si[j] * kAcc;
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+          if (src[k] != 0.0f) atomicAdd(dst + k, src[k]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launch
+template <int TW, int BATCH>
+static void fwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid) {
+  k_raster_fwd<TW, BATCH><<<grid, TW * TW, 0, s>>>(A);
+}
+template <int TW, int BATCH>
+static void bwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid, bool refine) {
+  if (refine)
+    k_raster_bwd<TW, BATCH, true><<<grid, TW * TW, 0, s>>>(A);
+  else
+    k_raster_bwd<TW, BATCH, false><<<grid, TW * TW, 0, s>>>(A);
+}
+
+#define RGS_DISPATCH(TWV, BV, CALL)                     \
+  if (tw == TWV && batch == BV) {                       \
+    CALL;                                               \
+    return 0;                                           \
+  }
+
+int launch_raster_fwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid) {
+  RGS_DISPATCH(8, 128, (fwd_launch<8, 128>(s, A, grid)))
+  RGS_DISPATCH(8, 256, (fwd_launch<8, 256>(s, A, grid)))
+  RGS_DISPATCH(8, 512, (fwd_launch<8, 512>(s, A, grid)))
+  RGS_DISPATCH(16, 128, (fwd_launch<16, 128>(s, A, grid)))
+  RGS_DISPATCH(16, 256, (fwd_launch<16, 256>(s, A, grid)))
+  RGS_DISPATCH(16, 512, (fwd_launch<16, 512>(s, A, grid)))
+  RGS_DISPATCH(32, 128, (fwd_launch<32, 128>(s, A, grid)))
+  RGS_DISPATCH(32, 256, (fwd_launch<32, 256>(s, A, grid)))
+  RGS_DISPATCH(32, 512, (fwd_launch<32, 512>(s, A, grid)))
+  return 1;
+}
+
+int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid, bool refine) {
+  RGS_DISPATCH(8, 128, (bwd_launch<8, 128>(s, A, grid, refine)))
+  RGS_DISPATCH(8, 256, (bwd_launch<8, 256>(s, A, grid, refine)))
+  RGS_DISPATCH(8, 512, (bwd_launch<8, 512>(s, A, grid, refine)))
+  RGS_DISPATCH(16, 128, (bwd_launch<16, 128>(s, A, grid, refine)))
+  RGS_DISPATCH(16, 256, (bwd_launch<16, 256>(s, A, grid, refine)))
+  RGS_DISPATCH(16, 512, (bwd_launch<16, 512>(s, A, grid, refine)))
+  RGS_DISPATCH(32, 128, (bwd_launch<32, 128>(s, A, grid, refine)))
+  RGS_DISPATCH(32, 256, (bwd_launch<32, 256>(s, A, grid, refine)))
+  RGS_DISPATCH(32, 512, (bwd_launch<32, 512>(s, A, grid, refine)))
+  return 1;
+}
+
+// Occupancy-based persistent grid size for a configuration.
+int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus) {
+  int per_cu = 0;
+  const int threads = tw * tw;
+#define RGS_OCC(TWV, BV)                                                                                     \
+  if (tw == TWV && batch == BV) {                                                                            \
+    if (bwd)                                                                                                 \
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_bwd<TWV, BV, false>, threads, 0); \
+    else                                                                                                     \
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_fwd<TWV, BV>, threads, 0);  \
+  }
+  RGS_OCC(8, 128) RGS_OCC(8, 256) RGS_OCC(8, 512)
+  RGS_OCC(16, 128) RGS_OCC(16, 256) RGS_OCC(16, 512)
+  RGS_OCC(32, 128) RGS_OCC(32, 256) RGS_OCC(32, 512)
+#undef RGS_OCC
+  if (per_cu < 1) per_cu = 1;
+  return per_cu * num_cus;
+}
+
+}  // namespace rgs

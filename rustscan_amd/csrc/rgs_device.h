@@ -1,0 +1,120 @@
+// Device-side building blocks shared by the projection, binning and raster
+// kernels.  The arithmetic of every function is written out operation by
+// operation (files are compiled without fp contraction) because the binning
+// decisions must be reproducible bit for bit; each cites the WGSL it computes.
+#pragma once
+
+#include "rgs_internal.h"
+
+namespace rgs {
+
+// Constants are the reference's WGSL abstract-float literals rounded to f32.
+#define RGS_F(x) ((float)(x))
+
+__device__ __forceinline__ float fminf_(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmaxf_(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float clampf_(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }
+
+// 2x2 matrix, column-major: c{col}{row}
+struct M2 {
+  float c00, c01, c10, c11;
+};
+
+__device__ __forceinline__ float det2(const M2& m) { return m.c00 * m.c11 - m.c01 * m.c10; }
+
+// helpers.wgsl:185-198 inverse2x2 (with the +1e-6 diagonal regulariser)
+__device__ __forceinline__ M2 inverse2x2(const M2& m) {
+  M2 r = m;
+  r.c00 = r.c00 + RGS_F(1e-6);
+  r.c11 = r.c11 + RGS_F(1e-6);
+  const float det = det2(r);
+  if (det <= RGS_F(1e-10)) return M2{0.f, 0.f, 0.f, 0.f};
+  const float inv = 1.0f / det;
+  return M2{r.c11 * inv, -r.c01 * inv, -r.c01 * inv, r.c00 * inv};
+}
+
+// helpers.wgsl:204-216 get_tile_bbox: (min_x, min_y, max_x, max_y) in tiles
+__device__ __forceinline__ void tile_bbox(float cx_, float cy_, float ex, float ey, uint32_t tbx, uint32_t tby,
+                                          int tw, uint32_t bb[4]) {
+  const float twf = (float)tw;
+  const float c0 = cx_ / twf, c1 = cy_ / twf;
+  const float e0 = ex / twf, e1 = ey / twf;
+  const float bx = (float)tbx, by = (float)tby;
+  bb[0] = (uint32_t)clampf_(c0 - e0, 0.0f, bx);
+  bb[1] = (uint32_t)clampf_(c1 - e1, 0.0f, by);
+  bb[2] = (uint32_t)clampf_(c0 + e0 + 1.0f, 0.0f, bx);
+  bb[3] = (uint32_t)clampf_(c1 + e1 + 1.0f, 0.0f, by);
+}
+
+// helpers.wgsl:218-224 compute_bbox_extent
+__device__ __forceinline__ void bbox_extent(float cov00, float cov11, float thr, float* ex, float* ey) {
+  const float safe = fmaxf_(thr, 0.0f);
+  *ex = rgs_sqrt(fmaxf_(0.0f, 2.0f * safe * cov00));
+  *ey = rgs_sqrt(fmaxf_(0.0f, 2.0f * safe * cov11));
+}
+
+// helpers.wgsl:232-235 calc_sigma (delta = pixel - mean)
+__device__ __forceinline__ float calc_sigma(float px, float py, float a, float b, float c, float mx, float my) {
+  const float dx = px - mx, dy = py - my;
+  return 0.5f * (a * dx * dx + 2.0f * b * dx * dy + c * dy * dy);
+}
+
+// helpers.wgsl:237-283 will_primitive_contribute for tile (tx, ty)
+__device__ __forceinline__ bool tile_contributes(uint32_t tx, uint32_t ty, int tw, float mx, float my, float a,
+                                                 float b, float c, float thr) {
+  const float rx0 = (float)(tx * (uint32_t)tw), ry0 = (float)(ty * (uint32_t)tw);
+  const float rx1 = rx0 + (float)tw, ry1 = ry0 + (float)tw;
+  const bool x_left = mx < rx0, x_right = mx > rx1;
+  const bool in_x = !(x_left || x_right);
+  const bool y_above = my < ry0, y_below = my > ry1;
+  const bool in_y = !(y_above || y_below);
+  if (in_x && in_y) return true;
+  const float ccx = x_left ? rx0 : rx1, ccy = y_above ? ry0 : ry1;
+  const float width = rx1 - rx0, height = ry1 - ry0;
+  const float dxr = x_left ? width : -width, dyr = y_above ? height : -height;
+  const float diffx = mx - ccx, diffy = my - ccy;
+  float t0 = 0.0f, t1 = 0.0f;
+  if (!in_y)
+    t0 = clampf_((dxr * a * diffx + dxr * b * diffy) / fmaxf_(dxr * a * dxr, RGS_F(1e-12)), 0.0f, 1.0f);
+  if (!in_x)
+    t1 = clampf_((dyr * b * diffx + dyr * c * diffy) / fmaxf_(dyr * c * dyr, RGS_F(1e-12)), 0.0f, 1.0f);
+  const float mpx = ccx + t0 * dxr, mpy = ccy + t1 * dyr;
+  return calc_sigma(mpx, mpy, a, b, c, mx, my) <= thr;
+}
+
+__device__ __forceinline__ float pack_u16x2(uint32_t lo, uint32_t hi) {
+  return __uint_as_float((lo & 0xffffu) | (hi << 16));
+}
+__device__ __forceinline__ void unpack_u16x2(float f, uint32_t* lo, uint32_t* hi) {
+  const uint32_t u = __float_as_uint(f);
+  *lo = u & 0xffffu;
+  *hi = u >> 16;
+}
+
+// Number of tiles of bbox bb passing the tile test (the K5 emission rule).
+__device__ __forceinline__ uint32_t count_tiles(const uint32_t bb[4], int tw, float mx, float my, float a, float b,
+                                                float c, float thr) {
+  const uint32_t bw = bb[2] - bb[0];
+  const uint32_t nb = (bb[3] - bb[1]) * bw;
+  uint32_t hit = 0;
+  for (uint32_t i = 0; i < nb; ++i) {
+    const uint32_t tx = bb[0] + i % bw, ty = bb[1] + i / bw;
+    if (tile_contributes(tx, ty, tw, mx, my, a, b, c, thr)) ++hit;
+  }
+  return hit;
+}
+
+__device__ __forceinline__ int find_view(const ViewDev* __restrict__ views, int nv, uint32_t tile) {
+  int v = 0;
+  for (int i = 1; i < nv; ++i)
+    if (tile >= views[i].tile_base) v = i;
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+}  // namespace rgs

@@ -1,0 +1,87 @@
+// Internal declarations shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rgs_math.h"
+
+namespace rgs {
+
+constexpr int kMaxViews = 64;
+constexpr int kMaxShDegree = 4;
+
+// Per-view constants.  Lives in device memory so a captured graph can be
+// replayed with new cameras after a plain host->device copy.
+struct ViewDev {
+  float vm[16];  // world->camera, column-major
+  float fx, fy, cx, cy;
+  uint32_t W, H;
+  float campos[3];
+  uint32_t tiles_x, tiles_y;
+  uint32_t tile_base;   // global id of this view's first tile
+  uint32_t pixel_base;  // offset of this view's first pixel in the image arena
+  // EWA clip limits of calc_cam_j (helpers.wgsl:144-145)
+  float lim_px, lim_py, lim_nx, lim_ny;
+  float bg[3];
+  float loss_scale;  // loss weight / (3*W*H)
+  // Per-view image pointers (device).  img: [H][W][4] output of the forward;
+  // target: [H][W][3] (fused L1); vout: [H][W][4] image gradient (unfused).
+  float* img;
+  const float* target;
+  const float* vout;
+};
+
+// Projected splat, 48 bytes, the unit every later stage gathers by instance.
+//   p0 = (x, y, a/2, b)        conic = [[a, b], [b, c]]
+//   p1 = (c/2, opacity, r, g)  colour before the max(., 0) clamp
+//   p2 = (blue, threshold, bbox_lo, bbox_hi)
+// threshold = log(255 * opacity) (the power threshold of the tile test); the
+// tile bbox is stored as two packed uint16 pairs (x, y) reinterpreted as floats.
+struct ProjSplat {
+  float4 p0, p1, p2;
+};
+
+// Counters block in device memory (one per workspace).
+enum Counter : int {
+  kCntVisible = 0,      // visible instances over all views
+  kCntIsect = 1,        // intersections emitted (may exceed capacity)
+  kCntOverflow = 2,     // nonzero if any capacity was exceeded this step
+  kCntQueueFwd = 3,     // persistent raster work counters
+  kCntQueueBwd = 4,
+  kCntStepValid = 5,    // 0 when the step must not update parameters
+  kCntAdamStep = 6,     // Adam step counter (all groups advance together)
+  kCntIsectMax = 7,     // max intersections of any forward in this step
+  kCntSkipped = 8,      // steps whose update was skipped
+  kCntNum = 16,
+};
+
+// Float scalars block
+enum FScalar : int {
+  kFLossBase = 0,  // [kMaxViews] per-view loss sums
+  kFBc1 = kMaxViews,
+  kFBc2,
+  kFNum = kMaxViews + 8,
+};
+
+constexpr int kAcc = 16;
+
+// Arguments of the compositing kernels.  Work items are global tile ids
+// [tile_first, tile_first + tile_count).
+struct RasterArgs {
+  const ProjSplat* proj;
+  const uint32_t* isect_vals;  // sorted instance ids
+  const uint32_t* ranges;      // [2 * tiles] (start, end) into isect_vals
+  const ViewDev* views;
+  int v0, nv;
+  uint32_t tile_first, tile_count;
+  uint32_t* queue;  // persistent work counter, or null for one workgroup per tile
+  uint32_t n;       // Gaussians per view; instance = view * n + gid
+  float* visible;   // [n] OR over views, or null
+  float* loss;      // per-view sums of |pred - target|, or null
+  float* acc;       // backward: [instances][16]
+};  // floats per instance in the raster-gradient accumulator
+
+inline int sh_coeffs(int degree) { return (degree + 1) * (degree + 1); }
+
+}  // namespace rgs

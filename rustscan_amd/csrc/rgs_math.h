@@ -1,0 +1,93 @@
+// Deterministic single-precision scalar math shared by the HIP kernels and
+// by host code.
+//
+// Every function here is a fixed sequence of IEEE-754 basic operations
+// (+, -, *, /, sqrt, fma, round-to-nearest-even, bit casts), each of which is
+// correctly rounded on both the CPU (SSE/FMA3) and the GPU (gfx950 with
+// correctly rounded fp32 divide/sqrt and no fp contraction).  The same inputs
+// therefore give the same bits on both sides, which is what makes the
+// visibility cull, the tile tests and the alpha thresholds of the renderer
+// reproducible between device and host.
+//
+// exp: Cody-Waite reduction x = k*ln2 + r with ln2 split in two parts, then
+//      the degree-7 Taylor polynomial of e^r on |r| <= ln2/2 (Horner, fma),
+//      scaled by 2^k built from its exponent bits.  Inputs <= -87 return 0,
+//      inputs > 88.7 return +inf, NaN propagates.
+// log: x = m * 2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+//      log(m) = 2*atanh(s) = 2s + 2s^3/3 + ... + 2s^11/11, plus e*ln2.
+//      Defined for positive normal x; x <= 0 returns -inf, NaN propagates,
+//      +inf returns +inf.
+// Accuracy against correctly rounded results is a few ulp over the full
+// float range; it is checked exhaustively-by-sampling in the test suite.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define RGS_HD __host__ __device__ __forceinline__
+#else
+#define RGS_HD static inline
+#endif
+
+RGS_HD uint32_t rgs_f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+RGS_HD float rgs_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+RGS_HD float rgs_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+RGS_HD float rgs_sqrt(float x) { return __builtin_sqrtf(x); }
+RGS_HD float rgs_rint(float x) { return __builtin_rintf(x); }
+
+RGS_HD float rgs_exp(float x) {
+  if (x != x) return x;
+  if (x <= -87.0f) return 0.0f;
+  if (x > 88.72283f) return rgs_u2f(0x7f800000u);
+  const float k = rgs_rint(x * 1.44269504088896341f);
+  // ln2 = LN2_HI + LN2_LO; LN2_HI has 15 significant bits so k*LN2_HI is exact.
+  float r = rgs_fma(-k, 0.693145751953125f, x);
+  r = rgs_fma(-k, 1.428606765330187045e-06f, r);
+  float p = 1.9841269841269841e-04f;            // 1/7!
+  p = rgs_fma(p, r, 1.3888888888888889e-03f);   // 1/6!
+  p = rgs_fma(p, r, 8.3333333333333333e-03f);   // 1/5!
+  p = rgs_fma(p, r, 4.1666666666666667e-02f);   // 1/4!
+  p = rgs_fma(p, r, 1.6666666666666667e-01f);   // 1/3!
+  p = rgs_fma(p, r, 0.5f);
+  p = rgs_fma(p, r, 1.0f);
+  p = rgs_fma(p, r, 1.0f);
+  const int ki = (int)k;  // in [-126, 128]
+  if (ki > 127) return (p * 2.0f) * rgs_u2f(0x7f000000u);
+  return p * rgs_u2f((uint32_t)(ki + 127) << 23);
+}
+
+RGS_HD float rgs_log(float x) {
+  if (x != x) return x;
+  if (x <= 0.0f) return rgs_u2f(0xff800000u);
+  const uint32_t bits = rgs_f2u(x);
+  if (bits >= 0x7f800000u) return x;
+  if (bits < 0x00800000u) {
+    // subnormal: scale into the normal range first (exact)
+    return rgs_log(x * 16777216.0f) - 16.635532333438687f;
+  }
+  int e = (int)((bits >> 23) & 0xffu) - 127;
+  float m = rgs_u2f((bits & 0x007fffffu) | 0x3f800000u);  // [1, 2)
+  if (m > 1.41421356237f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f;          // exact (Sterbenz)
+  const float s = f / (2.0f + f);
+  const float s2 = s * s;
+  float q = 0.18181818181818182f;                 // 2/11
+  q = rgs_fma(q, s2, 0.22222222222222222f);       // 2/9
+  q = rgs_fma(q, s2, 0.28571428571428571f);       // 2/7
+  q = rgs_fma(q, s2, 0.4f);                       // 2/5
+  q = rgs_fma(q, s2, 0.66666666666666667f);       // 2/3
+  // log(m) = 2s + s*s2*q
+  const float lm = rgs_fma(s * s2, q, 2.0f * s);
+  const float ef = (float)e;
+  return rgs_fma(ef, 0.693145751953125f, rgs_fma(ef, 1.428606765330187045e-06f, lm));
+}
+
+// 1 / (1 + e^-x), the logistic function used for opacities.
+RGS_HD float rgs_sigmoid(float x) { return 1.0f / (1.0f + rgs_exp(-x)); }
+
+// 1 / sqrt(x) as two correctly rounded operations.
+RGS_HD float rgs_rsqrt(float x) { return 1.0f / rgs_sqrt(x); }

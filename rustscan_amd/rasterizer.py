@@ -1,0 +1,329 @@
+"""Host-side mirror of RustGS's rasterizer/trainer interface over librgs_hip.
+
+Names follow the reference so parity tests read like its own tests:
+
+  render_forward                 training/forward/mod.rs:156-294
+  render_splats_with_visibility  training/backward/autodiff.rs:240-251
+  RenderBackward.backward        training/backward/autodiff.rs:85-141
+  AdamScaled                     training/engine/optimizer.rs:7-163
+  Trainer.train_step             training/engine/trainer.rs:253-440
+  lr_at                          training/engine/trainer.rs:179-191
+
+Device memory for images and targets is held in torch tensors on the HIP
+device; all computation happens in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from ctypes import c_float, c_int64, c_void_p
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .scene import Camera, Splats, sh_coeffs
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device available: the render path runs only on the GPU")
+    return torch
+
+
+def to_rgs_camera(cam: Camera) -> L.RgsCamera:
+    c = L.RgsCamera()
+    for i in range(16):
+        c.viewmat[i] = float(cam.viewmat[i])
+    c.fx, c.fy, c.cx, c.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
+    c.width, c.height = int(cam.width), int(cam.height)
+    for i in range(3):
+        c.cam_pos[i] = float(cam.cam_pos[i])
+    return c
+
+
+def _cams(cams) -> tuple:
+    arr = (L.RgsCamera * len(cams))(*[to_rgs_camera(c) for c in cams])
+    return arr, len(cams)
+
+
+def _ptrs(tensors) -> ctypes.Array:
+    return (c_void_p * len(tensors))(*[t.data_ptr() if t is not None else None for t in tensors])
+
+
+class Context:
+    """One HIP stream and workspace (rgs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        _torch()
+        self.lib = L.lib()
+        h = c_void_p()
+        L.check(None, self.lib.rgs_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.rgs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, code):
+        L.check(self.h, code)
+
+    def sync(self):
+        self.check(self.lib.rgs_sync(self.h))
+
+    def set_raster(self, tile_width=16, batch_depth=256, persistent=True):
+        cfg = L.RgsRasterCfg(tile_width, batch_depth, 1 if persistent else 0)
+        self.check(self.lib.rgs_set_raster_config(self.h, ctypes.byref(cfg)))
+
+    def raster(self) -> tuple:
+        cfg = L.RgsRasterCfg()
+        self.check(self.lib.rgs_get_raster_config(self.h, ctypes.byref(cfg)))
+        return cfg.tile_width, cfg.batch_depth, bool(cfg.persistent)
+
+    def debug(self, what: int, view: int = 0, dtype=np.uint32) -> np.ndarray:
+        cnt = c_int64()
+        self.check(self.lib.rgs_debug_get(self.h, what, view, None, ctypes.byref(cnt)))
+        out = np.zeros(cnt.value, dtype)
+        if cnt.value:
+            self.check(self.lib.rgs_debug_get(self.h, what, view, out.ctypes.data, ctypes.byref(cnt)))
+        return out
+
+    def counts(self, n_views: int):
+        arr = (L.RgsCounts * n_views)()
+        self.check(self.lib.rgs_get_counts(self.h, arr, n_views))
+        return [(a.num_visible, a.num_intersections, a.num_tiles) for a in arr]
+
+    def skipped_steps(self) -> int:
+        return int(self.lib.rgs_skipped_steps(self.h))
+
+    def set_debug(self, keep_raster_grads: bool):
+        self.check(self.lib.rgs_set_debug(self.h, 1 if keep_raster_grads else 0))
+
+
+class DeviceSplats:
+    """GPU-resident splat store (DeviceSplats, engine/splats.rs:13-63)."""
+
+    def __init__(self, ctx: Context, host: Splats):
+        self.ctx = ctx
+        self.n = host.n
+        self.sh_degree = host.sh_degree
+        self.k = sh_coeffs(host.sh_degree)
+        h = c_void_p()
+        ctx.check(ctx.lib.rgs_splats_create(ctx.h, self.n, self.sh_degree, ctypes.byref(h)))
+        self.h = h
+        self.upload(host)
+
+    def upload(self, host: Splats):
+        tf = np.ascontiguousarray(host.transforms, np.float32)
+        sh = np.ascontiguousarray(host.sh, np.float32)
+        op = np.ascontiguousarray(host.raw_opac, np.float32)
+        self.ctx.check(self.ctx.lib.rgs_splats_import_aos(self.ctx.h, self.h, tf.ctypes.data, sh.ctypes.data,
+                                                          op.ctypes.data))
+
+    def download(self) -> Splats:
+        tf = np.zeros((self.n, 10), np.float32)
+        sh = np.zeros((self.n, self.k, 3), np.float32)
+        op = np.zeros(self.n, np.float32)
+        self.ctx.check(self.ctx.lib.rgs_splats_export_aos(self.ctx.h, self.h, tf.ctypes.data, sh.ctypes.data,
+                                                          op.ctypes.data))
+        return Splats(tf, sh, op, self.sh_degree)
+
+    def grads(self):
+        vt = np.zeros((self.n, 10), np.float32)
+        vs = np.zeros((self.n, self.k, 3), np.float32)
+        vo = np.zeros(self.n, np.float32)
+        st = np.zeros((self.n, 7), np.float32)
+        self.ctx.check(self.ctx.lib.rgs_splats_export_grads(self.ctx.h, self.h, vt.ctypes.data, vs.ctypes.data,
+                                                            vo.ctypes.data, st.ctypes.data))
+        return vt, vs, vo, st
+
+    def visible(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        self.ctx.check(self.ctx.lib.rgs_splats_export_visible(self.ctx.h, self.h, out.ctypes.data))
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.rgs_splats_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_forward(ctx: Context, splats: DeviceSplats, cams, background=(0.0, 0.0, 0.0), cov_blur: float = 0.3):
+    """Renders every camera; returns [H,W,4] float32 torch tensors on the device."""
+    torch = _torch()
+    arr, nv = _cams(cams)
+    outs = [torch.empty((c.height, c.width, 4), dtype=torch.float32, device="cuda") for c in cams]
+    bg = (c_float * 3)(*background)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_render_forward(ctx.h, splats.h, arr, nv, bg, float(cov_blur), _ptrs(outs)))
+    return outs
+
+
+def render_backward(ctx: Context, splats: DeviceSplats, v_outs):
+    """RenderBackward::backward: image gradients -> splat-store gradients."""
+    torch = _torch()
+    v_outs = [v.contiguous().float() for v in v_outs]
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_render_backward(ctx.h, splats.h, _ptrs(v_outs)))
+    return splats.grads()
+
+
+def l1_loss_grad(ctx: Context, pred, target, weight: float = 1.0):
+    torch = _torch()
+    h, w = int(pred.shape[0]), int(pred.shape[1])
+    v_out = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    loss = c_float()
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_l1_loss_grad(ctx.h, pred.data_ptr(), target.contiguous().data_ptr(), w, h, float(weight),
+                                       v_out.data_ptr(), ctypes.byref(loss)))
+    return loss.value, v_out
+
+
+@dataclass
+class AdamScaledConfig:
+    """engine/optimizer.rs:7-24 (lr 1.0 in the trainer, per-column scaling)."""
+
+    lr: float = 1.0
+    betas: tuple = (0.9, 0.999)
+    eps: float = 1e-8
+    weight_decay: float = 0.0
+
+
+@dataclass
+class AdamScaled:
+    config: AdamScaledConfig = field(default_factory=AdamScaledConfig)
+    transform_scaling: list = field(default_factory=lambda: [1.0] * 10)
+    sh_scaling: list = field(default_factory=lambda: [1.0] * 25)
+    opacity_scaling: float = 1.0
+
+    def set_transform_scaling(self, s):
+        self.transform_scaling = [float(x) for x in s]
+
+    def set_sh_scaling(self, s):
+        self.sh_scaling = [float(x) for x in s] + [0.0] * (25 - len(s))
+
+    def set_opacity_scaling(self, s: float):
+        self.opacity_scaling = float(s)
+
+    def to_cfg(self) -> L.RgsAdamCfg:
+        if self.config.weight_decay != 0.0:
+            raise ValueError("weight decay is not supported by the fused Adam (the trainer uses 0)")
+        c = L.RgsAdamCfg()
+        c.beta1, c.beta2, c.eps = float(np.float32(self.config.betas[0])), float(np.float32(self.config.betas[1])), \
+            float(np.float32(self.config.eps))
+        lr = np.float32(self.config.lr)
+        for i in range(10):
+            c.lr_transforms[i] = float(np.float32(self.transform_scaling[i]) * lr)
+        for i in range(25):
+            c.lr_sh[i] = float(np.float32(self.sh_scaling[i]) * lr)
+        c.lr_opacity = float(np.float32(self.opacity_scaling) * lr)
+        return c
+
+    def step_device_splats(self, ctx: Context, splats: DeviceSplats):
+        cfg = self.to_cfg()
+        ctx.check(ctx.lib.rgs_adam_step(ctx.h, splats.h, ctypes.byref(cfg)))
+
+    def reset(self, ctx: Context, splats: DeviceSplats):
+        ctx.check(ctx.lib.rgs_adam_reset(ctx.h, splats.h))
+
+
+def lr_at(initial: float, final_value: float, iteration: int, iterations: int, decay_iterations=None) -> float:
+    """engine/trainer.rs:179-191, in float32 like the reference."""
+    if final_value <= 0.0 or final_value >= initial or iterations == 0:
+        return initial
+    d = max(decay_iterations if decay_iterations is not None else iterations, 1)
+    t = np.float32(min(iteration, d)) / np.float32(d)
+    return float(np.float32(initial) * np.exp(np.float32(np.log(np.float32(final_value) / np.float32(initial))) * t,
+                                               dtype=np.float32))
+
+
+def training_optimizer(sh_degree: int, lr_position: float, lr_rotation: float, lr_scale: float, lr_opacity: float,
+                       lr_sh_dc: float, lr_sh_rest: float | None = None) -> AdamScaled:
+    """The per-column learning rates of WgpuTrainer::new (trainer.rs:116-151);
+    lr_sh_rest, when given, sets the scaling of the non-DC SH coefficients."""
+    opt = AdamScaled()
+    opt.set_transform_scaling([lr_position] * 3 + [lr_rotation] * 4 + [lr_scale] * 3)
+    k = sh_coeffs(sh_degree)
+    rest = lr_sh_dc if lr_sh_rest is None else lr_sh_rest
+    opt.set_sh_scaling([lr_sh_dc] + [rest] * (k - 1))
+    opt.set_opacity_scaling(lr_opacity)
+    return opt
+
+
+class Trainer:
+    """Fused training step (WgpuTrainer::train_step) over one or more views."""
+
+    def __init__(self, ctx: Context, optimizer: AdamScaled, cov_blur: float = 0.3, background=(0.0, 0.0, 0.0),
+                 loss_weight: float = 1.0, check_loss: bool = True):
+        self.ctx = ctx
+        self.optimizer = optimizer
+        self.cov_blur = cov_blur
+        self.background = background
+        self.loss_weight = loss_weight
+        self.check_loss = check_loss
+
+    def step_cfg(self, batched: bool, graph: bool) -> L.RgsStepCfg:
+        c = L.RgsStepCfg()
+        c.mode = L.RGS_MODE_BATCHED if batched else L.RGS_MODE_PER_VIEW
+        c.graph = L.RGS_GRAPH_ON if graph else L.RGS_GRAPH_OFF
+        c.cov_blur = float(self.cov_blur)
+        for i in range(3):
+            c.background[i] = float(self.background[i])
+        c.loss_weight = float(self.loss_weight)
+        c.check_loss = 1 if self.check_loss else 0
+        c.adam = self.optimizer.to_cfg()
+        return c
+
+    def train_step(self, splats: DeviceSplats, cams, targets, batched: bool = True, graph: bool = False,
+                   read_loss: bool = False):
+        arr, nv = _cams(cams)
+        tgts = _ptrs(targets)
+        cfg = self.step_cfg(batched, graph)
+        loss = (c_float * nv)() if read_loss else None
+        self.ctx.check(self.ctx.lib.rgs_train_step(self.ctx.h, splats.h, arr, tgts, nv, ctypes.byref(cfg), loss))
+        return list(loss) if read_loss else None
+
+    def autotune(self, splats: DeviceSplats, cams, targets, batched: bool = True, graph: bool = True):
+        arr, nv = _cams(cams)
+        cfg = self.step_cfg(batched, graph)
+        out = L.RgsRasterCfg()
+        self.ctx.check(self.ctx.lib.rgs_autotune(self.ctx.h, splats.h, arr, _ptrs(targets), nv, ctypes.byref(cfg),
+                                                 ctypes.byref(out)))
+        return out.tile_width, out.batch_depth, bool(out.persistent)
+
+
+def sort_pairs_u32(ctx: Context, keys, values):
+    """radix_sort_by_key_u32_primitive on device tensors (int32 views of u32)."""
+    torch = _torch()
+    n = int(keys.numel())
+    ok = torch.empty_like(keys)
+    ov = torch.empty_like(values)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_sort_pairs_u32(ctx.h, keys.data_ptr() if n else None, values.data_ptr() if n else None,
+                                         ok.data_ptr() if n else None, ov.data_ptr() if n else None, n))
+    return ok, ov
+
+
+def inclusive_scan_u32(ctx: Context, x):
+    torch = _torch()
+    n = int(x.numel())
+    out = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_inclusive_scan_u32(ctx.h, x.data_ptr() if n else None, out.data_ptr() if n else None, n))
+    return out

@@ -1,0 +1,124 @@
+"""HIP render path against the oracle through the C ABI.
+
+Forward (config 1, and config 2 at reduced and full size): depth order, per-
+Gaussian tile counts, sorted intersections and tile ranges bit-exact under the
+canonical tie rule; projected splats and images within 1e-4 (the shared
+deterministic math makes them equal in practice, the maximum is printed);
+visibility flags equal.  Backward: per-group gradients within 1e-3 relative
+L2 of the oracle's reference-semantics backward.
+"""
+import numpy as np
+import pytest
+
+from rustscan_amd import scene
+from tests.gpu_util import rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+
+def render_both(hip_ctx, orc, sc, cam, blur=0.3, tile=16, batch=256):
+    from rustscan_amd.rasterizer import DeviceSplats, render_forward
+
+    hip_ctx.set_raster(tile, batch, True)
+    ds = DeviceSplats(hip_ctx, sc)
+    imgs = render_forward(hip_ctx, ds, [cam], cov_blur=blur)
+    r = orc.Render(sc, cam, cov_blur=blur, tile_width=tile)
+    return ds, imgs[0].cpu().numpy(), r
+
+
+def check_forward(hip_ctx, orc, sc, cam, tile=16, batch=256):
+    from rustscan_amd import _lib as L
+
+    ds, img, r = render_both(hip_ctx, orc, sc, cam, tile=tile, batch=batch)
+    ref = r.image()
+    err = float(np.abs(img - ref).max())
+    say(f"  image max abs err {err:.3g} (V={r.stats['num_visible']}, M={r.stats['num_intersections']})")
+    assert err <= 1e-4
+    order = hip_ctx.debug(L.RGS_DEBUG_DEPTH_ORDER)
+    np.testing.assert_array_equal(order, r.depth_order())
+    np.testing.assert_array_equal(hip_ctx.debug(L.RGS_DEBUG_TILE_COUNTS), r.counts())
+    tiles, gids = r.isect()
+    np.testing.assert_array_equal(hip_ctx.debug(L.RGS_DEBUG_ISECT_TILES), tiles)
+    np.testing.assert_array_equal(hip_ctx.debug(L.RGS_DEBUG_ISECT_GIDS), gids)
+    np.testing.assert_array_equal(hip_ctx.debug(L.RGS_DEBUG_TILE_RANGES), r.tile_offsets())
+    proj = hip_ctx.debug(L.RGS_DEBUG_PROJECTED, dtype=np.float32).reshape(-1, 9)
+    perr = float(np.abs(proj[r.depth_order()] - r.projected()).max())
+    say(f"  projected max abs err {perr:.3g}")
+    assert perr <= 1e-4 * max(1.0, float(np.abs(r.projected()).max()))
+    np.testing.assert_array_equal(ds.visible(), r.visible())
+    return ds, img, r
+
+
+def test_forward_single_gaussian(hip_ctx, orc):
+    # the reference's forward test scene (forward/tests.rs:8-39): one Gaussian at z=2, 64x64, f=500
+    import math
+
+    tf = np.array([[0, 0, 2, 1, 0, 0, 0, math.log(0.2), math.log(0.2), math.log(0.2)]], np.float32)
+    sh = np.array([[[(c - 0.5) / 0.28209479177387814 for c in (1.0, 0.5, 0.25)]]], np.float32)
+    sc = scene.Splats(tf, sh, np.zeros(1, np.float32), 0)
+    say("forward single gaussian")
+    check_forward(hip_ctx, orc, sc, scene.pinhole_identity(64, 64, 500.0))
+
+
+def test_forward_config1(hip_ctx, orc):
+    sc, cams = scene.config1()
+    say("forward config1")
+    check_forward(hip_ctx, orc, sc, cams[0])
+
+
+def test_forward_config2_small(hip_ctx, orc):
+    sc, cams, _ = scene.config2(n=16384, width=320, height=240)
+    say("forward config2 small")
+    check_forward(hip_ctx, orc, sc, cams[0])
+
+
+def test_forward_config2_full(hip_ctx, orc):
+    sc, cams, _ = scene.config2()
+    say("forward config2 full")
+    check_forward(hip_ctx, orc, sc, cams[0])
+
+
+def grads_vs_oracle(hip_ctx, orc, sc, cam, target, tile=16, batch=256, tol=1e-3):
+    import torch
+
+    from rustscan_amd.rasterizer import l1_loss_grad, render_backward
+
+    ds, img, r = render_both(hip_ctx, orc, sc, cam, tile=tile, batch=batch)
+    tgt = torch.from_numpy(target).cuda()
+    loss, v_out = l1_loss_grad(hip_ctx, torch.from_numpy(img).cuda(), tgt)
+    ref_loss, ref_vout = orc.l1_loss_grad(r.image(), target)
+    assert abs(loss - ref_loss) <= 1e-5 * max(1.0, abs(ref_loss))
+    np.testing.assert_array_equal(v_out.cpu().numpy(), ref_vout)
+    vt, vs, vo, st = render_backward(hip_ctx, ds, [v_out])
+    ref = r.backward(ref_vout, mode=0)
+    errs = {
+        "means": rel_l2(vt[:, 0:3], ref["v_transforms"][:, 0:3]),
+        "quats": rel_l2(vt[:, 3:7], ref["v_transforms"][:, 3:7]),
+        "log_scales": rel_l2(vt[:, 7:10], ref["v_transforms"][:, 7:10]),
+        "opacity": rel_l2(vo, ref["v_raw_opac"]),
+        "sh": rel_l2(vs, ref["v_sh"]),
+        "screen_stats": rel_l2(st, ref["screen_stats"]),
+    }
+    say("  grad rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    for k, v in errs.items():
+        assert v <= tol, f"{k}: rel L2 {v:.3e}"
+    return errs
+
+
+def test_backward_config1(hip_ctx, orc):
+    sc, cams = scene.config1()
+    target = scene.Rng(3).uniform(320 * 240 * 3).reshape(240, 320, 3).astype(np.float32)
+    say("backward config1")
+    grads_vs_oracle(hip_ctx, orc, sc, cams[0], target)
+
+
+def test_backward_config2_small(hip_ctx, orc):
+    sc, cams, tg = scene.config2(n=16384, width=320, height=240)
+    say("backward config2 small")
+    grads_vs_oracle(hip_ctx, orc, sc, cams[0], tg[0])
+
+
+def test_backward_config2_full(hip_ctx, orc):
+    sc, cams, tg = scene.config2()
+    say("backward config2 full")
+    grads_vs_oracle(hip_ctx, orc, sc, cams[0], tg[0])
