@@ -131,8 +131,10 @@ int rgs_adam_reset(rgs_ctx* ctx, rgs_splats* s);
  * project_visible.rs:36-45), depth sort (SortingBackend, sorting.rs:8-61), tile
  * binning (TileMappingBackend, tile_mapping.rs:82-95), compositing
  * (RasterizeBackend, rasterize.rs:48-59) for n_views cameras in one batch.
- * out_rgba: n_views device pointers (H*W*4 floats each) or NULL to keep the
- * images internal.  Keeps the forward state for rgs_render_backward. */
+ * out_rgba: n_views pointers (H*W*4 floats each) that receive copies of the
+ * images, or NULL.  The library renders into its own buffers and keeps them,
+ * with the rest of the forward state, for rgs_render_backward
+ * (RenderCheckpoint, autodiff.rs:56-71).  Synchronous. */
 int rgs_render_forward(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, int n_views, const float* background,
                        float cov_blur, float* const* out_rgba);
 
@@ -199,8 +201,33 @@ enum {
 };
 int rgs_debug_get(rgs_ctx* ctx, int what, int view, void* dst, int64_t* count);
 /* flags bit 0: keep a copy of the rasterizer accumulators of each backward
- * (RGS_DEBUG_RASTER_GRADS); costs one extra copy per backward. */
+ * (RGS_DEBUG_RASTER_GRADS); costs one extra copy per backward.
+ * flags bit 1: count compositing fragments in eager launches. */
 int rgs_set_debug(rgs_ctx* ctx, int flags);
+/* Fragment counts since the last reset (bit 1 of rgs_set_debug): [0] forward
+ * (pixel, splat) pairs evaluated, [1] forward pairs blended, [2] backward pairs
+ * evaluated, [3] backward pairs contributing a gradient.  Synchronous. */
+int rgs_get_fragment_counts(rgs_ctx* ctx, int64_t* out4, int reset);
+
+/* ---- per-stage timing (HIP events on the context stream) ----------------
+ * When enabled, eager (non-graph) launches record events around each stage;
+ * rgs_get_timing synchronises and returns the accumulated milliseconds and
+ * launch counts per stage (arrays of RGS_STAGE_COUNT). */
+enum {
+  RGS_STAGE_PROJECT_FWD = 0,
+  RGS_STAGE_DEPTH_SORT = 1,
+  RGS_STAGE_COUNT_SCAN = 2,
+  RGS_STAGE_EMIT = 3,
+  RGS_STAGE_TILE_SORT = 4,
+  RGS_STAGE_TILE_RANGES = 5,
+  RGS_STAGE_RASTER_FWD = 6,
+  RGS_STAGE_RASTER_BWD = 7,
+  RGS_STAGE_PROJECT_BWD = 8,
+  RGS_STAGE_ADAM = 9,
+  RGS_STAGE_COUNT = 10
+};
+int rgs_set_timing(rgs_ctx* ctx, int enable);
+int rgs_get_timing(rgs_ctx* ctx, double* ms, int64_t* calls, int reset);
 
 /* Library version string. */
 const char* rgs_version(void);

@@ -32,6 +32,9 @@ RGS_DEBUG_TILE_COUNTS = 5
 RGS_DEBUG_IMAGE = 6
 RGS_DEBUG_RASTER_GRADS = 7
 
+STAGES = ["project_fwd", "depth_sort", "count_scan", "emit", "tile_sort", "tile_ranges", "raster_fwd", "raster_bwd",
+          "project_bwd", "adam"]
+
 
 class RgsCamera(ctypes.Structure):
     _fields_ = [
@@ -118,6 +121,9 @@ _SIGNATURES = {
     "rgs_inclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
     "rgs_debug_get": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(c_int64)]),
     "rgs_set_debug": (c_int, [c_void_p, c_int]),
+    "rgs_get_fragment_counts": (c_int, [c_void_p, POINTER(c_int64), c_int]),
+    "rgs_set_timing": (c_int, [c_void_p, c_int]),
+    "rgs_get_timing": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_int64), c_int]),
     "rgs_version": (ctypes.c_char_p, []),
 }
 

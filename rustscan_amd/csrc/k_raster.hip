@@ -73,6 +73,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !inside;
+    uint32_t n_eval = 0, n_blend = 0;
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
@@ -88,6 +89,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
       __syncthreads();
       if (!done) {
         for (uint32_t j = 0; j < cnt; ++j) {
+          ++n_eval;
           const float4 a0 = s0[j];
           const float dx = a0.x - pcx, dy = a0.y - pcy;
           const float4 a1 = s1[j];
@@ -96,6 +98,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
           if (sigma > a2.y) continue;
           const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp(-sigma));
           if (sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0)) {
+            ++n_blend;
             const float vis = alpha * T;
             cr = cr + a1.z * vis;
             cg = cg + a1.w * vis;
@@ -127,6 +130,18 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
     if (A.loss) {
       const float s = wave_sum(l1);
       if ((tid & 63) == 0 && s != 0.0f) atomicAdd(&A.loss[v], s);
+    }
+    if (A.frag) {
+      unsigned long long e = n_eval, bl = n_blend;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        e += __shfl_xor(e, o, 64);
+        bl += __shfl_xor(bl, o, 64);
+      }
+      if ((tid & 63) == 0) {
+        atomicAdd(&A.frag[0], e);
+        atomicAdd(&A.frag[1], bl);
+      }
     }
   }
 }
@@ -181,6 +196,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
     float T = 1.0f;
     float po[3] = {0.f, 0.f, 0.f};
     bool done = !inside;
+    uint32_t n_eval = 0, n_contrib = 0;
     const float Wf = (float)V.W, Hf = (float)V.H;
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
@@ -202,6 +218,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         for (int k = 0; k < 12; ++k) val[k] = 0.0f;
         bool contrib = false;
         if (!done) {
+          ++n_eval;
           const float4 a0 = s0[j];
           const float4 a1 = s1[j];
           const float2 a2 = s2[j];
@@ -253,6 +270,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
             }
           }
         }
+        n_contrib += contrib ? 1u : 0u;
         const uint64_t m = __ballot(contrib);
         if (m == 0) continue;
 #pragma unroll
@@ -276,6 +294,18 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
 #pragma unroll
         for (int k = 0; k < NA; ++k)
           if (src[k] != 0.0f) atomicAdd(dst + k, src[k]);
+      }
+    }
+    if (A.frag) {
+      unsigned long long e = n_eval, cb2 = n_contrib;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        e += __shfl_xor(e, o, 64);
+        cb2 += __shfl_xor(cb2, o, 64);
+      }
+      if ((tid & 63) == 0) {
+        atomicAdd(&A.frag[2], e);
+        atomicAdd(&A.frag[3], cb2);
       }
     }
   }

@@ -136,6 +136,19 @@ struct rgs_ctx {
   uint32_t* h_counters = nullptr;
   float* h_fscal = nullptr;
   std::map<std::vector<uint32_t>, rgs_raster_cfg> tuned;
+  // per-stage timing
+  bool timing = false;
+  bool capturing = false;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double stage_ms[RGS_STAGE_COUNT] = {};
+  int64_t stage_calls[RGS_STAGE_COUNT] = {};
+  // fragment counters (device, 4 x u64) when enabled
+  DevBuf frag;
+  bool count_frags = false;
 };
 
 namespace {
@@ -176,6 +189,26 @@ void release(DevBuf& b) {
   b.p = nullptr;
   b.bytes = 0;
 }
+
+// Records HIP events around one stage when timing is on (never while capturing).
+struct StageTimer {
+  rgs_ctx* c;
+  int stage;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(rgs_ctx* ctx, int st) : c(ctx), stage(st) {
+    if (c->timing && !c->capturing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (a) {
+      hipEventRecord(b, c->stream);
+      c->pending.push_back({stage, a, b});
+    }
+  }
+};
 
 void invalidate_graph(rgs_ctx* c) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -335,26 +368,46 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
 
   launch_fwd_begin(st, counters);
   launch_fill_u32(st, w.ranges.as<uint32_t>() + 2 * (size_t)tile_first, 0u, 2 * (size_t)tile_count);
-  launch_project_fwd(s->deg, st, s->P.as<float>(), s->npad, n, views, v0, nv, blur, vs.tw, w.proj.as<ProjSplat>(),
-                     w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), w.cnt.as<uint32_t>(), counters);
+  {
+    StageTimer t(c, RGS_STAGE_PROJECT_FWD);
+    launch_project_fwd(s->deg, st, s->P.as<float>(), s->npad, n, views, v0, nv, blur, vs.tw, w.proj.as<ProjSplat>(),
+                       w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), w.cnt.as<uint32_t>(), counters);
+  }
   // depth sort of all instances of the batch (sorting.rs:34-61; canonical ties: instance order)
-  const int wd = radix_sort_pairs(st, w.keys.as<uint32_t>() + ibase, w.vals.as<uint32_t>() + ibase,
-                                  w.sk1.as<uint32_t>(), w.sv1.as<uint32_t>(), w.sk2.as<uint32_t>(),
-                                  w.sv2.as<uint32_t>(), nullptr, nitems, nitems, 32, w.scratch.as<uint32_t>());
+  int wd;
+  {
+    StageTimer t(c, RGS_STAGE_DEPTH_SORT);
+    wd = radix_sort_pairs(st, w.keys.as<uint32_t>() + ibase, w.vals.as<uint32_t>() + ibase, w.sk1.as<uint32_t>(),
+                          w.sv1.as<uint32_t>(), w.sk2.as<uint32_t>(), w.sv2.as<uint32_t>(), nullptr, nitems, nitems,
+                          32, w.scratch.as<uint32_t>());
+  }
   uint32_t* order = wd == 1 ? w.sv1.as<uint32_t>() : w.sv2.as<uint32_t>();
   // exclusive scan of per-instance tile counts in depth order (prefix_sum.rs:53-136)
-  scan_u32(st, w.cnt.as<uint32_t>(), order, counters + kCntVisible, 0, nitems, w.offs.as<uint32_t>(), 0,
-           counters + kCntIsect, counters + kCntIsectMax, w.scratch.as<uint32_t>());
+  {
+    StageTimer t(c, RGS_STAGE_COUNT_SCAN);
+    scan_u32(st, w.cnt.as<uint32_t>(), order, counters + kCntVisible, 0, nitems, w.offs.as<uint32_t>(), 0,
+             counters + kCntIsect, counters + kCntIsectMax, w.scratch.as<uint32_t>());
+  }
   // emission (map_gaussian_to_intersects.wgsl) and stable tile sort (tile_mapping.rs:206-215)
-  launch_emit(st, w.proj.as<ProjSplat>(), order, w.offs.as<uint32_t>(), counters, n, views, vs.tw, nitems,
-              (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
+  {
+    StageTimer t(c, RGS_STAGE_EMIT);
+    launch_emit(st, w.proj.as<ProjSplat>(), order, w.offs.as<uint32_t>(), counters, n, views, vs.tw, nitems,
+                (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
+  }
   const int bits = (int)ceil_log2(tile_end);
-  const int wt = radix_sort_pairs(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(),
-                                  w.iv1.as<uint32_t>(), w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(),
-                                  counters + kCntIsect, 0, (uint32_t)w.cap_isect, bits, w.scratch.as<uint32_t>());
+  int wt;
+  {
+    StageTimer t(c, RGS_STAGE_TILE_SORT);
+    wt = radix_sort_pairs(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(), w.iv1.as<uint32_t>(),
+                          w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect, 0, (uint32_t)w.cap_isect,
+                          bits, w.scratch.as<uint32_t>());
+  }
   uint32_t* skeys = wt == 1 ? w.ik1.as<uint32_t>() : w.ik0.as<uint32_t>();
   uint32_t* svals = wt == 1 ? w.iv1.as<uint32_t>() : w.iv0.as<uint32_t>();
-  launch_tile_ranges(st, skeys, counters, (uint32_t)w.cap_isect, tile_end, w.ranges.as<uint32_t>());
+  {
+    StageTimer t(c, RGS_STAGE_TILE_RANGES);
+    launch_tile_ranges(st, skeys, counters, (uint32_t)w.cap_isect, tile_end, w.ranges.as<uint32_t>());
+  }
   // compositing (rasterize.rs:67-119)
   RasterArgs A{};
   A.proj = w.proj.as<ProjSplat>();
@@ -370,6 +423,8 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   A.visible = s->visible.as<float>();
   A.loss = with_loss ? w.fscal.as<float>() + kFLossBase : nullptr;
   A.acc = nullptr;
+  A.frag = (c->count_frags && !c->capturing) ? c->frag.as<unsigned long long>() : nullptr;
+  StageTimer tr(c, RGS_STAGE_RASTER_FWD);
   if (tile_count > 0 &&
       launch_raster_fwd(st, (int)c->raster.tile_width, (int)c->raster.batch_depth, A, raster_grid(c, false, tile_count)))
     return fail(c, RGS_ERR_UNSUPPORTED, "raster configuration %ux%u/%u not built", c->raster.tile_width,
@@ -405,16 +460,21 @@ int enqueue_backward(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, int v0, int n
   A.visible = nullptr;
   A.loss = nullptr;
   A.acc = w.acc.as<float>();
+  A.frag = (c->count_frags && !c->capturing) ? c->frag.as<unsigned long long>() : nullptr;
   launch_set_u32(st, counters + kCntQueueBwd, 0u);
-  if (tile_count > 0 && launch_raster_bwd(st, (int)c->raster.tile_width, (int)c->raster.batch_depth, A,
-                                          raster_grid(c, true, tile_count), refine))
-    return fail(c, RGS_ERR_UNSUPPORTED, "raster configuration not built");
+  {
+    StageTimer t(c, RGS_STAGE_RASTER_BWD);
+    if (tile_count > 0 && launch_raster_bwd(st, (int)c->raster.tile_width, (int)c->raster.batch_depth, A,
+                                            raster_grid(c, true, tile_count), refine))
+      return fail(c, RGS_ERR_UNSUPPORTED, "raster configuration not built");
+  }
   if (c->keep_raster_grads) {
     const size_t bytes = (size_t)n * nv * kAcc * 4;
     if (ensure(c, w.acc_dbg, (size_t)w.cap_inst * kAcc * 4)) return RGS_ERR_HIP;
     HIPCHK(c, hipMemcpyAsync(w.acc_dbg.as<float>() + (size_t)v0 * n * kAcc, w.acc.as<float>() + (size_t)v0 * n * kAcc,
                              bytes, hipMemcpyDeviceToDevice, st));
   }
+  StageTimer tp(c, RGS_STAGE_PROJECT_BWD);
   launch_project_bwd(s->deg, st, s->P.as<float>(), s->npad, n, w.views.as<ViewDev>(), v0, nv, blur, w.acc.as<float>(),
                      w.keys.as<uint32_t>(), s->G.as<float>(), s->stats.as<float>(), accumulate ? 1 : 0, counters);
   return RGS_OK;
@@ -428,6 +488,7 @@ void set_lr(rgs_splats* s, const rgs_adam_cfg* cfg, float* host_planes) {
 }
 
 int enqueue_adam(rgs_ctx* c, rgs_splats* s, const rgs_adam_cfg* cfg, int nviews, int check_loss) {
+  StageTimer t(c, RGS_STAGE_ADAM);
   launch_adam(c->stream, s->P.as<float>(), s->G.as<float>(), s->M.as<float>(), s->V.as<float>(), s->npad,
               (uint32_t)s->n, s->nplanes, s->lr.as<float>(), cfg->beta1, cfg->beta2, cfg->eps, c->ws.fscal.as<float>(),
               c->ws.counters.as<uint32_t>(), nviews, check_loss);
@@ -541,6 +602,7 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   hipStreamSynchronize(c->stream);
   invalidate_graph(c);
   Workspace& w = c->ws;
+  release(c->frag);
   DevBuf* bufs[] = {&w.views, &w.proj, &w.keys, &w.vals, &w.cnt, &w.sk1, &w.sv1, &w.sk2, &w.sv2, &w.offs, &w.ik0,
                     &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch};
   for (DevBuf* b : bufs) release(*b);
@@ -724,7 +786,9 @@ int rgs_render_forward(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, int nv
   ViewSet vs;
   int rc = build_views(c, cams, nv, bg, 1.0f, (int)c->raster.tile_width, vs);
   if (rc) return rc;
-  if ((rc = attach_images(c, vs, out_rgba))) return rc;
+  // Render into library-owned images: the backward reads them later, and a
+  // caller's buffer may be released or reused in between.
+  if ((rc = attach_images(c, vs, nullptr))) return rc;
   if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, 0))) return rc;
   blur = blur > 0.0f ? blur : 0.0f;
   for (int attempt = 0; attempt < 3; ++attempt) {
@@ -737,6 +801,13 @@ int rgs_render_forward(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, int nv
     if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, (size_t)c->h_counters[kCntIsectMax] * 5 / 4 + 1024)))
       return rc;
     if (attempt == 2) return fail(c, RGS_ERR_CAPACITY, "intersection buffer could not be sized");
+  }
+  if (out_rgba) {
+    for (int i = 0; i < nv; ++i)
+      if (out_rgba[i])
+        HIPCHK(c, hipMemcpyAsync(out_rgba[i], vs.v[i].img, (size_t)vs.v[i].W * vs.v[i].H * 16,
+                                 hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   c->fwd_valid = true;
   c->fwd_splats = s;
@@ -837,7 +908,9 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
       if (!reuse) {
         invalidate_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         rc = enqueue_step(c, s, vs, cfg);
+        c->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
@@ -1136,10 +1209,59 @@ int rgs_debug_get(rgs_ctx* c, int what, int view, void* dst, int64_t* count) {
   }
 }
 
+int rgs_set_timing(rgs_ctx* c, int enable) {
+  if (!c) return RGS_ERR_INVALID;
+  c->timing = enable != 0;
+  return RGS_OK;
+}
+
+int rgs_get_timing(rgs_ctx* c, double* ms, int64_t* calls, int reset) {
+  if (!c) return RGS_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+      c->stage_ms[p.stage] += t;
+      c->stage_calls[p.stage] += 1;
+    }
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  c->pending.clear();
+  for (int i = 0; i < RGS_STAGE_COUNT; ++i) {
+    if (ms) ms[i] = c->stage_ms[i];
+    if (calls) calls[i] = c->stage_calls[i];
+    if (reset) {
+      c->stage_ms[i] = 0.0;
+      c->stage_calls[i] = 0;
+    }
+  }
+  return RGS_OK;
+}
+
 int rgs_set_debug(rgs_ctx* c, int flags) {
   if (!c) return RGS_ERR_INVALID;
   c->keep_raster_grads = (flags & 1) != 0;
+  c->count_frags = (flags & 2) != 0;
+  if (c->count_frags && !c->frag.p) {
+    if (ensure(c, c->frag, 4 * sizeof(unsigned long long))) return RGS_ERR_HIP;
+    HIPCHK(c, hipMemsetAsync(c->frag.p, 0, 4 * sizeof(unsigned long long), c->stream));
+  }
   invalidate_graph(c);
+  return RGS_OK;
+}
+
+int rgs_get_fragment_counts(rgs_ctx* c, int64_t* out4, int reset) {
+  if (!c || !out4) return RGS_ERR_INVALID;
+  if (!c->frag.p) {
+    for (int i = 0; i < 4; ++i) out4[i] = 0;
+    return RGS_OK;
+  }
+  unsigned long long h[4];
+  HIPCHK(c, hipMemcpyAsync(h, c->frag.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 4; ++i) out4[i] = (int64_t)h[i];
+  if (reset) HIPCHK(c, hipMemsetAsync(c->frag.p, 0, sizeof(h), c->stream));
   return RGS_OK;
 }
 

@@ -80,6 +80,9 @@ struct RasterArgs {
   float* visible;   // [n] OR over views, or null
   float* loss;      // per-view sums of |pred - target|, or null
   float* acc;       // backward: [instances][16]
+  // optional fragment counters: forward [0] evaluated, [1] blended;
+  // backward [2] evaluated, [3] contributing
+  unsigned long long* frag;
 };  // floats per instance in the raster-gradient accumulator
 
 inline int sh_coeffs(int degree) { return (degree + 1) * (degree + 1); }

@@ -106,8 +106,25 @@ class Context:
     def skipped_steps(self) -> int:
         return int(self.lib.rgs_skipped_steps(self.h))
 
-    def set_debug(self, keep_raster_grads: bool):
-        self.check(self.lib.rgs_set_debug(self.h, 1 if keep_raster_grads else 0))
+    def set_timing(self, enable: bool):
+        self.check(self.lib.rgs_set_timing(self.h, 1 if enable else 0))
+
+    def timing(self, reset: bool = True) -> dict:
+        """Accumulated per-stage milliseconds and launch counts (eager launches only)."""
+        n = len(L.STAGES)
+        ms = (ctypes.c_double * n)()
+        calls = (c_int64 * n)()
+        self.check(self.lib.rgs_get_timing(self.h, ms, calls, 1 if reset else 0))
+        return {name: (ms[i], calls[i]) for i, name in enumerate(L.STAGES)}
+
+    def set_debug(self, keep_raster_grads: bool = False, count_fragments: bool = False):
+        flags = (1 if keep_raster_grads else 0) | (2 if count_fragments else 0)
+        self.check(self.lib.rgs_set_debug(self.h, flags))
+
+    def fragment_counts(self, reset: bool = True) -> dict:
+        out = (c_int64 * 4)()
+        self.check(self.lib.rgs_get_fragment_counts(self.h, out, 1 if reset else 0))
+        return dict(fwd_evaluated=out[0], fwd_blended=out[1], bwd_evaluated=out[2], bwd_contributing=out[3])
 
 
 class DeviceSplats:
