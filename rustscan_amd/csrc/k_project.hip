@@ -145,11 +145,12 @@ __device__ __forceinline__ void project_cov(const Geom& g, const ViewDev& V, flo
 #pragma unroll
     for (int r = 0; r < 3; ++r)
       A[c][r] = (V.vm[0 * 4 + r] * g.cov[c][0] + V.vm[1 * 4 + r] * g.cov[c][1]) + V.vm[2 * 4 + r] * g.cov[c][2];
+  // (A * R^T)[c][r] = sum_k A[k][r] * R^T[c][k] = sum_k A[k][r] * R[k][c], R[k][c] = vm[k*4 + c]
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < 3; ++r)
-      o.cc[c][r] = (A[0][r] * V.vm[c * 4 + 0] + A[1][r] * V.vm[c * 4 + 1]) + A[2][r] * V.vm[c * 4 + 2];
+      o.cc[c][r] = (A[0][r] * V.vm[0 * 4 + c] + A[1][r] * V.vm[1 * 4 + c]) + A[2][r] * V.vm[2 * 4 + c];
   const float rz = 1.0f / fmaxf_(mcz, RGS_F(0.01));
   const float ux = clampf_(mcx * rz, V.lim_nx, V.lim_px);
   const float uy = clampf_(mcy * rz, V.lim_ny, V.lim_py);
