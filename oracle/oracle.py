@@ -83,6 +83,7 @@ def lib():
                                          POINTER(OrcCamera), c_void_p, c_float, c_void_p, c_void_p]
         L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
+        L.orc_math_exp_core.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_sh_color.argtypes = [c_int, c_void_p, c_void_p, c_void_p]
         L.orc_sort_by_key.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
         L.orc_inclusive_scan.argtypes = [c_int64, c_void_p, c_void_p]
@@ -271,6 +272,13 @@ def math_exp(x: np.ndarray) -> np.ndarray:
     x = _f32(x)
     y = np.zeros_like(x)
     lib().orc_math_exp(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def math_exp_core(x: np.ndarray) -> np.ndarray:
+    x = _f32(x)
+    y = np.zeros_like(x)
+    lib().orc_math_exp_core(x.ctypes.data, y.ctypes.data, x.size)
     return y
 
 

@@ -1398,6 +1398,9 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
 void orc_math_exp(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_exp(x[i]);
 }
+void orc_math_exp_core(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = rgs_exp_core(x[i]);
+}
 void orc_math_log(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_log(x[i]);
 }

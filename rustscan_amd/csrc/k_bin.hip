@@ -247,38 +247,78 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_down(const uint32_t* __re
 // ------------------------------------------------------------- emission
 // map_gaussian_to_intersects.wgsl:15-49: for each visible instance in depth
 // order, write (global tile id, instance) for every tile of its bbox that
-// passes the tile test, at offs[i] + k.  Writes past `cap` are dropped and
-// flagged; the step is then marked invalid so no parameter is updated.
+// passes the tile test, at offs[i] + k.  The 256 instances of a workgroup own
+// one contiguous output range, which is assembled in LDS and written out
+// coalesced (ranges too large for LDS are written directly).  Bboxes of at most
+// 64 tiles reuse the pass mask of the projection pass instead of re-testing.
+// Writes past `cap` are dropped and flagged; the step is then marked invalid
+// so no parameter is updated.
+constexpr uint32_t kEmitStage = 4096;
+
 __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj, const uint32_t* __restrict__ order,
-                                              const uint32_t* __restrict__ offs, uint32_t* __restrict__ counters,
-                                              uint32_t n_per_view, const ViewDev* __restrict__ views, int tw,
-                                              uint32_t cap_inst, uint32_t cap, uint32_t* __restrict__ okeys,
+                                              const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnt,
+                                              uint32_t* __restrict__ counters, uint32_t n_per_view,
+                                              const ViewDev* __restrict__ views, int tw, uint32_t cap_inst,
+                                              uint32_t cap, uint32_t* __restrict__ okeys,
                                               uint32_t* __restrict__ ovals) {
+  __shared__ uint32_t sk[kEmitStage], sv[kEmitStage];
+  __shared__ uint32_t s_base, s_total;
+  __shared__ int s_over;
   const uint32_t V = min(counters[kCntVisible], cap_inst);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V) return;
-  const uint32_t inst = order[i];
-  const uint32_t v = inst / n_per_view;
-  const ViewDev& VD = views[v];
-  const ProjSplat s = proj[inst];
-  uint32_t bb[4];
-  unpack_u16x2(s.p2.z, &bb[0], &bb[1]);
-  unpack_u16x2(s.p2.w, &bb[2], &bb[3]);
-  const float a = 2.0f * s.p0.z, b = s.p0.w, c = 2.0f * s.p1.x, thr = s.p2.y;
-  const uint32_t bw = bb[2] - bb[0];
-  const uint32_t nb = (bb[3] - bb[1]) * bw;
-  uint32_t pos = offs[i];
+  const uint32_t i0 = blockIdx.x * blockDim.x;
+  if (i0 >= V) return;  // whole workgroup
+  if (threadIdx.x == 0) {
+    const uint32_t last = min(i0 + blockDim.x, V) - 1;
+    s_base = offs[i0];
+    s_total = offs[last] + cnt[order[last]] - offs[i0];
+    s_over = 0;
+  }
+  __syncthreads();
+  const uint32_t base = s_base, total = s_total;
+  const bool staged = total <= kEmitStage;
+  const uint32_t i = i0 + threadIdx.x;
   bool over = false;
-  for (uint32_t k = 0; k < nb; ++k) {
-    const uint32_t tx = bb[0] + k % bw, ty = bb[1] + k / bw;
-    if (tile_contributes(tx, ty, tw, s.p0.x, s.p0.y, a, b, c, thr)) {
-      if (pos < cap) {
-        okeys[pos] = VD.tile_base + tx + ty * VD.tiles_x;
-        ovals[pos] = inst;
+  if (i < V) {
+    const uint32_t inst = order[i];
+    const uint32_t v = inst / n_per_view;
+    const ViewDev& VD = views[v];
+    const ProjSplat s = proj[inst];
+    uint32_t bb[4];
+    unpack_u16x2(s.p2.z, &bb[0], &bb[1]);
+    unpack_u16x2(s.p2.w, &bb[2], &bb[3]);
+    const float a = 2.0f * s.p0.z, b = s.p0.w, c = 2.0f * s.p1.x, thr = s.p2.y;
+    const uint32_t nb = (bb[3] - bb[1]) * (bb[2] - bb[0]);
+    const uint64_t mask = (uint64_t)__float_as_uint(s.p3.x) | ((uint64_t)__float_as_uint(s.p3.y) << 32);
+    uint32_t pos = offs[i];
+    uint32_t k = 0;
+    for (uint32_t ty = bb[1]; ty < bb[3]; ++ty)
+      for (uint32_t tx = bb[0]; tx < bb[2]; ++tx, ++k) {
+        const bool hit = nb <= 64 ? ((mask >> k) & 1ull) != 0
+                                  : tile_contributes(tx, ty, tw, s.p0.x, s.p0.y, a, b, c, thr);
+        if (!hit) continue;
+        const uint32_t key = VD.tile_base + tx + ty * VD.tiles_x;
+        if (staged) {
+          sk[pos - base] = key;
+          sv[pos - base] = inst;
+        } else if (pos < cap) {
+          okeys[pos] = key;
+          ovals[pos] = inst;
+        } else {
+          over = true;
+        }
+        ++pos;
+      }
+  }
+  if (staged) {
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const uint32_t p = base + e;
+      if (p < cap) {
+        okeys[p] = sk[e];
+        ovals[p] = sv[e];
       } else {
         over = true;
       }
-      ++pos;
     }
   }
   if (over) {
@@ -386,11 +426,11 @@ void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const u
 }
 
 void launch_emit(hipStream_t s, const ProjSplat* proj, const uint32_t* order, const uint32_t* offs,
-                 uint32_t* counters, uint32_t n_per_view, const ViewDev* views, int tw, uint32_t cap_inst,
-                 uint32_t cap, uint32_t* okeys, uint32_t* ovals) {
+                 const uint32_t* cnt, uint32_t* counters, uint32_t n_per_view, const ViewDev* views, int tw,
+                 uint32_t cap_inst, uint32_t cap, uint32_t* okeys, uint32_t* ovals) {
   if (cap_inst == 0) return;
-  k_emit<<<(cap_inst + 255) / 256, 256, 0, s>>>(proj, order, offs, counters, n_per_view, views, tw, cap_inst, cap,
-                                                 okeys, ovals);
+  k_emit<<<(cap_inst + 255) / 256, 256, 0, s>>>(proj, order, offs, cnt, counters, n_per_view, views, tw, cap_inst,
+                                                 cap, okeys, ovals);
 }
 
 void launch_tile_ranges(hipStream_t s, const uint32_t* keys, const uint32_t* counters, uint32_t cap,

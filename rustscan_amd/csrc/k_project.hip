@@ -233,7 +233,8 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
             bbox_extent(cv.c00, cv.c11, thr, &ex5, &ey5);
             uint32_t bb[4];
             tile_bbox(m2x, m2y, ex5, ey5, V.tiles_x, V.tiles_y, tw, bb);
-            count = count_tiles(bb, tw, m2x, m2y, a, b, c, thr);
+            uint64_t mask;
+            count = count_tiles(bb, tw, m2x, m2y, a, b, c, thr, &mask);
             // colour
             const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
             const float dd = (dx * dx + dy * dy) + dz * dz;
@@ -244,6 +245,7 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
             ps.p0 = make_float4(m2x, m2y, 0.5f * a, b);
             ps.p1 = make_float4(0.5f * c, opacity, col[0] + 0.5f, col[1] + 0.5f);
             ps.p2 = make_float4(col[2] + 0.5f, thr, pack_u16x2(bb[0], bb[1]), pack_u16x2(bb[2], bb[3]));
+            ps.p3 = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.0f, 0.0f);
             proj[inst] = ps;
             keys[inst] = __float_as_uint(mcz);
           }

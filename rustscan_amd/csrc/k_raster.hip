@@ -50,6 +50,20 @@ __device__ __forceinline__ int view_of(const RasterArgs& A, uint32_t tile) {
   return v;
 }
 
+// Sum over the 64 lanes of a wave with DPP row shifts and row broadcasts;
+// the total is valid in lane 63.
+#define RGS_DPP(v, ctrl, rm, bm) \
+  __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, (v)), ctrl, rm, bm, true))
+__device__ __forceinline__ float wave_reduce_l63(float v) {
+  v += RGS_DPP(v, 0x111, 0xf, 0xf);  // row_shr:1
+  v += RGS_DPP(v, 0x112, 0xf, 0xf);  // row_shr:2
+  v += RGS_DPP(v, 0x114, 0xf, 0xf);  // row_shr:4
+  v += RGS_DPP(v, 0x118, 0xf, 0xf);  // row_shr:8  -> lane 15 of each row holds the row sum
+  v += RGS_DPP(v, 0x142, 0xa, 0xf);  // row_bcast:15 into rows 1 and 3
+  v += RGS_DPP(v, 0x143, 0xc, 0xf);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 template <int TW, int BATCH>
 __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
   constexpr int NT = TW * TW;
@@ -87,29 +101,29 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
         sv[j] = 0u;
       }
       __syncthreads();
-      if (!done) {
-        for (uint32_t j = 0; j < cnt; ++j) {
-          ++n_eval;
-          const float4 a0 = s0[j];
-          const float dx = a0.x - pcx, dy = a0.y - pcy;
-          const float4 a1 = s1[j];
-          const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
-          const float2 a2 = s2[j];
-          if (sigma > a2.y) continue;
-          const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp(-sigma));
-          if (sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0)) {
-            ++n_blend;
-            const float vis = alpha * T;
-            cr = cr + a1.z * vis;
-            cg = cg + a1.w * vis;
-            cb = cb + a2.x * vis;
-            T = T * (1.0f - alpha);
-            sv[j] = 1u;
-            if (T < RGS_F(1e-4)) {
-              done = true;
-              break;
-            }
-          }
+      // Wave-uniform loop: every lane walks the batch until the whole wave is
+      // done; a finished lane only stops updating its pixel.
+      for (uint32_t j = 0; j < cnt; ++j) {
+        if (__all(done)) break;
+        const float4 a0 = s0[j];
+        const float4 a1 = s1[j];
+        const float2 a2 = s2[j];
+        const float dx = a0.x - pcx, dy = a0.y - pcy;
+        const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
+        n_eval += done ? 0u : 1u;
+        const bool need = !done && !(sigma > a2.y);
+        if (!__any(need)) continue;
+        const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core(-sigma));
+        const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
+        if (acc) {
+          ++n_blend;
+          const float vis = alpha * T;
+          cr = cr + a1.z * vis;
+          cg = cg + a1.w * vis;
+          cb = cb + a2.x * vis;
+          T = T * (1.0f - alpha);
+          sv[j] = 1u;
+          if (T < RGS_F(1e-4)) done = true;
         }
       }
       __syncthreads();
@@ -157,6 +171,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
   __shared__ float sacc[BATCH * NA];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   uint32_t iter = 0, item;
   while (next_item(A, &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item;
@@ -213,72 +228,68 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
       __syncthreads();
       for (uint32_t j = 0; j < cnt; ++j) {
         if (__all(done)) break;
+        const float4 a0 = s0[j];
+        const float4 a1 = s1[j];
+        const float2 a2 = s2[j];
+        const float dx = a0.x - pcx, dy = a0.y - pcy;
+        const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
+        n_eval += done ? 0u : 1u;
+        const bool need = !done && !(sigma > a2.y);
+        if (!__any(need)) continue;
+        const float gauss = rgs_exp_core(-sigma);
+        const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
+        const bool pass = need && !(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0));
+        const float nT = T * (1.0f - alpha);
+        const bool stop = pass && nT <= RGS_F(1e-4);
+        const bool contrib = pass && !stop;
+        done = done || stop;
+        const uint64_t m = __ballot(contrib);
+        if (m == 0) continue;
         float val[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) val[k] = 0.0f;
-        bool contrib = false;
-        if (!done) {
-          ++n_eval;
-          const float4 a0 = s0[j];
-          const float4 a1 = s1[j];
-          const float2 a2 = s2[j];
-          const float dx = a0.x - pcx, dy = a0.y - pcy;
-          const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
-          if (!(sigma > a2.y)) {
-            const float gauss = rgs_exp(-sigma);
-            const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
-            if (!(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0))) {
-              const float nT = T * (1.0f - alpha);
-              if (nT <= RGS_F(1e-4)) {
-                done = true;
-              } else {
-                contrib = true;
-                const float col[3] = {a1.z, a1.w, a2.x};
-                const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
-                const float vis = alpha * T;
+        if (contrib) {
+          ++n_contrib;
+          const float col[3] = {a1.z, a1.w, a2.x};
+          const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
+          const float vis = alpha * T;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
-                  po[c] = po[c] + vis * cl[c];
-                }
-                const float ra = 1.0f / (1.0f - alpha);
-                float tv[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
-                const float v_alpha = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
-                const float v_sigma = -alpha * v_alpha;
-                const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
-                const float vx = v_sigma * (ca * dx + cb * dy);
-                const float vy = v_sigma * (cb * dx + cc * dy);
-                if (a1.y * gauss <= RGS_F(0.999)) {
-                  val[0] = vx;
-                  val[1] = vy;
-                  val[2] = 0.5f * v_sigma * dx * dx;
-                  val[3] = v_sigma * dx * dy;
-                  val[4] = 0.5f * v_sigma * dy * dy;
-                  val[8] = v_alpha * gauss;
-                  if (REFINE) {
-                    const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
-                    const float lx = vx * Wf, ly = vy * Hf;
-                    val[9] = rgs_sqrt(lx * lx + ly * ly) / fa;
-                  }
-                  val[10] = fabsf(vx);
-                  val[11] = fabsf(vy);
-                }
-                T = nT;
-              }
-            }
+          for (int c = 0; c < 3; ++c) {
+            val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
+            po[c] = po[c] + vis * cl[c];
           }
+          const float ra = 1.0f / (1.0f - alpha);
+          float tv[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
+          const float v_alpha = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
+          const float v_sigma = -alpha * v_alpha;
+          const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
+          const float vx = v_sigma * (ca * dx + cb * dy);
+          const float vy = v_sigma * (cb * dx + cc * dy);
+          if (a1.y * gauss <= RGS_F(0.999)) {
+            val[0] = vx;
+            val[1] = vy;
+            val[2] = 0.5f * v_sigma * dx * dx;
+            val[3] = v_sigma * dx * dy;
+            val[4] = 0.5f * v_sigma * dy * dy;
+            val[8] = v_alpha * gauss;
+            if (REFINE) {
+              const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
+              const float lx = vx * Wf, ly = vy * Hf;
+              val[9] = rgs_sqrt(lx * lx + ly * ly) / fa;
+            }
+            val[10] = fabsf(vx);
+            val[11] = fabsf(vy);
+          }
+          T = nT;
         }
-        n_contrib += contrib ? 1u : 0u;
-        const uint64_t m = __ballot(contrib);
-        if (m == 0) continue;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
           if (k == 9 && !REFINE) continue;
-          val[k] = wave_sum(val[k]);
+          val[k] = wave_reduce_l63(val[k]);
         }
-        if ((tid & 63) == 0) {
+        if (lane == 63) {
           float* dst = sacc + j * NA;
 #pragma unroll
           for (int k = 0; k < 12; ++k)

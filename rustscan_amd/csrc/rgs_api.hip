@@ -32,8 +32,8 @@ void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const u
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev,
               uint32_t* scratch);
 void launch_emit(hipStream_t s, const ProjSplat* proj, const uint32_t* order, const uint32_t* offs,
-                 uint32_t* counters, uint32_t n_per_view, const ViewDev* views, int tw, uint32_t cap_inst,
-                 uint32_t cap, uint32_t* okeys, uint32_t* ovals);
+                 const uint32_t* cnt, uint32_t* counters, uint32_t n_per_view, const ViewDev* views, int tw,
+                 uint32_t cap_inst, uint32_t cap, uint32_t* okeys, uint32_t* ovals);
 void launch_tile_ranges(hipStream_t s, const uint32_t* keys, const uint32_t* counters, uint32_t cap,
                         uint32_t num_tiles, uint32_t* ranges);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
@@ -391,8 +391,8 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   // emission (map_gaussian_to_intersects.wgsl) and stable tile sort (tile_mapping.rs:206-215)
   {
     StageTimer t(c, RGS_STAGE_EMIT);
-    launch_emit(st, w.proj.as<ProjSplat>(), order, w.offs.as<uint32_t>(), counters, n, views, vs.tw, nitems,
-                (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
+    launch_emit(st, w.proj.as<ProjSplat>(), order, w.offs.as<uint32_t>(), w.cnt.as<uint32_t>(), counters, n, views,
+                vs.tw, nitems, (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
   }
   const int bits = (int)ceil_log2(tile_end);
   int wt;

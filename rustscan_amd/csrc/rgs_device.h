@@ -91,16 +91,23 @@ __device__ __forceinline__ void unpack_u16x2(float f, uint32_t* lo, uint32_t* hi
   *hi = u >> 16;
 }
 
-// Number of tiles of bbox bb passing the tile test (the K5 emission rule).
+// Tiles of bbox bb passing the tile test (the K5 emission rule), visited in
+// the reference's row-major order (map_gaussian_to_intersects.wgsl:37-40).
+// For bboxes of at most 64 tiles the passing set is returned as a bit mask.
 __device__ __forceinline__ uint32_t count_tiles(const uint32_t bb[4], int tw, float mx, float my, float a, float b,
-                                                float c, float thr) {
+                                                float c, float thr, uint64_t* mask) {
   const uint32_t bw = bb[2] - bb[0];
   const uint32_t nb = (bb[3] - bb[1]) * bw;
   uint32_t hit = 0;
-  for (uint32_t i = 0; i < nb; ++i) {
-    const uint32_t tx = bb[0] + i % bw, ty = bb[1] + i / bw;
-    if (tile_contributes(tx, ty, tw, mx, my, a, b, c, thr)) ++hit;
-  }
+  uint64_t m = 0;
+  uint32_t k = 0;
+  for (uint32_t ty = bb[1]; ty < bb[3]; ++ty)
+    for (uint32_t tx = bb[0]; tx < bb[2]; ++tx, ++k)
+      if (tile_contributes(tx, ty, tw, mx, my, a, b, c, thr)) {
+        ++hit;
+        if (k < 64) m |= 1ull << k;
+      }
+  *mask = nb <= 64 ? m : 0ull;
   return hit;
 }
 

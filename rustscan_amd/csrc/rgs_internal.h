@@ -32,14 +32,18 @@ struct ViewDev {
   const float* vout;
 };
 
-// Projected splat, 48 bytes, the unit every later stage gathers by instance.
+// Projected splat, 64 bytes (half a cache line), the unit every later stage
+// gathers by instance.
 //   p0 = (x, y, a/2, b)        conic = [[a, b], [b, c]]
 //   p1 = (c/2, opacity, r, g)  colour before the max(., 0) clamp
 //   p2 = (blue, threshold, bbox_lo, bbox_hi)
+//   p3 = (mask_lo, mask_hi, -, -)
 // threshold = log(255 * opacity) (the power threshold of the tile test); the
 // tile bbox is stored as two packed uint16 pairs (x, y) reinterpreted as floats.
+// mask: for bboxes of at most 64 tiles, bit k (row-major within the bbox) is
+// set when tile k passes the tile test; the emission pass reuses it.
 struct ProjSplat {
-  float4 p0, p1, p2;
+  float4 p0, p1, p2, p3;
 };
 
 // Counters block in device memory (one per workspace).

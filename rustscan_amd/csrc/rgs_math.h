@@ -57,6 +57,30 @@ RGS_HD float rgs_exp(float x) {
   return p * rgs_u2f((uint32_t)(ki + 127) << 23);
 }
 
+// rgs_exp without its special cases, for arguments clamped to [-87, 88]:
+// there k = rint(x*log2(e)) lies in [-126, 127], the scale 2^k is a normal
+// float and the result equals rgs_exp(x) bit for bit for every x in
+// (-87, 88] (at x = -87 rgs_exp returns 0, this returns 2^-126 * p).  The
+// compositing kernels only consume values for x = -sigma in [-5.6, 0], where
+// both agree; for other lanes the value is discarded.  NaN input returns a
+// finite value, which callers reject through their own sigma test.
+RGS_HD float rgs_exp_core(float x) {
+  x = x > -87.0f ? x : -87.0f;
+  x = x < 88.0f ? x : 88.0f;
+  const float k = rgs_rint(x * 1.44269504088896341f);
+  float r = rgs_fma(-k, 0.693145751953125f, x);
+  r = rgs_fma(-k, 1.428606765330187045e-06f, r);
+  float p = 1.9841269841269841e-04f;
+  p = rgs_fma(p, r, 1.3888888888888889e-03f);
+  p = rgs_fma(p, r, 8.3333333333333333e-03f);
+  p = rgs_fma(p, r, 4.1666666666666667e-02f);
+  p = rgs_fma(p, r, 1.6666666666666667e-01f);
+  p = rgs_fma(p, r, 0.5f);
+  p = rgs_fma(p, r, 1.0f);
+  p = rgs_fma(p, r, 1.0f);
+  return p * rgs_u2f((uint32_t)((int)k + 127) << 23);
+}
+
 RGS_HD float rgs_log(float x) {
   if (x != x) return x;
   if (x <= 0.0f) return rgs_u2f(0xff800000u);

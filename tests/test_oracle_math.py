@@ -56,3 +56,18 @@ def test_log_special(orc):
     assert np.isneginf(y[0]) and np.isneginf(y[1])
     assert np.isinf(y[2]) and y[2] > 0
     assert np.isnan(y[3])
+
+
+def test_exp_core_equals_exp(orc):
+    """The branch-free exp used by the compositing kernels equals rgs_exp bit
+    for bit on (-87, 88], in particular on the compositing range [-5.6, 0]."""
+    lo = np.float32(-5.6)
+    # every 53rd float in [-5.6, -0] (negative floats are ordered by their bit
+    # patterns), about 20 million arguments
+    b0 = np.float32(-0.0).view(np.uint32)
+    b1 = lo.view(np.uint32)
+    x = np.arange(b0, b1 + 1, 53, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    np.testing.assert_array_equal(orc.math_exp_core(x), orc.math_exp(x))
+    rng = np.random.default_rng(13)
+    y = rng.uniform(-86.99, 88.0, 2_000_000).astype(np.float32)
+    np.testing.assert_array_equal(orc.math_exp_core(y), orc.math_exp(y))
