@@ -20,8 +20,8 @@ __host__ __device__ inline float powi_f32(float a, int b) {
 // Advances the Adam step counter (all three parameter groups of AdamScaled
 // advance together, optimizer.rs:144) and computes the f32 bias corrections,
 // unless this step was invalidated (capacity overflow or non-finite loss).
-__global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restrict__ fscal, float b1, float b2,
-                                int nviews, int check_loss) {
+__global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restrict__ fscal,
+                                uint32_t* __restrict__ step_ctr, float b1, float b2, int nviews, int check_loss) {
   if (threadIdx.x != 0) return;
   if (check_loss) {
     for (int v = 0; v < nviews; ++v) {
@@ -30,8 +30,8 @@ __global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restri
     }
   }
   if (counters[kCntStepValid] == 0u) return;
-  const uint32_t step = counters[kCntAdamStep] + 1u;
-  counters[kCntAdamStep] = step;
+  const uint32_t step = *step_ctr + 1u;
+  *step_ctr = step;
   fscal[kFBc1] = 1.0f - powi_f32(b1, (int)step);
   fscal[kFBc2] = 1.0f - powi_f32(b2, (int)step);
 }
@@ -120,9 +120,9 @@ static uint32_t blocks_for(size_t n, uint32_t cap = 16384) {
 }
 
 void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, uint32_t npad, uint32_t n, int nplanes,
-                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters, int nviews,
-                 int check_loss) {
-  k_adam_prologue<<<1, 64, 0, s>>>(counters, fscal, b1, b2, nviews, check_loss);
+                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
+                 uint32_t* step, int nviews, int check_loss) {
+  k_adam_prologue<<<1, 64, 0, s>>>(counters, fscal, step, b1, b2, nviews, check_loss);
   k_adam<<<blocks_for((size_t)nplanes * npad), 256, 0, s>>>(P, G, M, V, npad, n, nplanes, lr_planes, b1, b2, eps,
                                                             fscal, counters);
 }

@@ -45,8 +45,8 @@ int launch_raster_fwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uin
 int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid, bool refine);
 int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus);
 void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, uint32_t npad, uint32_t n, int nplanes,
-                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters, int nviews,
-                 int check_loss);
+                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
+                 uint32_t* step, int nviews, int check_loss);
 void launch_l1(hipStream_t s, const float* pred, const float* tgt, uint32_t np, float scale, float* v_out,
                float* loss_sum);
 void launch_aos_to_soa(hipStream_t s, const float* aos, int width, uint32_t n, float* soa, uint32_t npad, int plane0);
@@ -77,6 +77,7 @@ struct rgs_splats {
   uint32_t npad = 0;
   int deg = 0, nk = 1, nplanes = 11;
   DevBuf P, G, M, V, stats, visible, lr;
+  DevBuf step;  // Adam step counter of this store (AdamState::step, optimizer.rs:27-32)
 };
 
 struct Workspace {
@@ -491,7 +492,7 @@ int enqueue_adam(rgs_ctx* c, rgs_splats* s, const rgs_adam_cfg* cfg, int nviews,
   StageTimer t(c, RGS_STAGE_ADAM);
   launch_adam(c->stream, s->P.as<float>(), s->G.as<float>(), s->M.as<float>(), s->V.as<float>(), s->npad,
               (uint32_t)s->n, s->nplanes, s->lr.as<float>(), cfg->beta1, cfg->beta2, cfg->eps, c->ws.fscal.as<float>(),
-              c->ws.counters.as<uint32_t>(), nviews, check_loss);
+              c->ws.counters.as<uint32_t>(), s->step.as<uint32_t>(), nviews, check_loss);
   return RGS_OK;
 }
 
@@ -663,6 +664,7 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   rc |= ensure(c, s->stats, (size_t)s->npad * 4 * 7);
   rc |= ensure(c, s->visible, (size_t)s->npad * 4);
   rc |= ensure(c, s->lr, 4 * (size_t)s->nplanes);
+  rc |= ensure(c, s->step, 4);
   if (rc) {
     rgs_splats_destroy(c, s);
     return RGS_ERR_HIP;
@@ -675,6 +677,7 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   launch_fill_f32(c->stream, s->stats.as<float>(), 0.0f, (size_t)s->npad * 7);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
   launch_fill_f32(c->stream, s->lr.as<float>(), 0.0f, s->nplanes);
+  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, 1);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rgs_splats_destroy(c, s);
     return fail(c, RGS_ERR_HIP, "stream error");
@@ -690,7 +693,7 @@ void rgs_splats_destroy(rgs_ctx* c, rgs_splats* s) {
     if (c->gsplats == s) invalidate_graph(c);
     if (c->fwd_splats == s) c->fwd_valid = false;
   }
-  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr};
+  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr, &s->step};
   for (DevBuf* b : bufs) release(*b);
   delete s;
 }
@@ -705,10 +708,8 @@ int rgs_adam_reset(rgs_ctx* c, rgs_splats* s) {
   const size_t nf = (size_t)s->npad * s->nplanes;
   launch_fill_f32(c->stream, s->M.as<float>(), 0.0f, nf);
   launch_fill_f32(c->stream, s->V.as<float>(), 0.0f, nf);
-  // step counter lives in the context counters block
+  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, 1);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  uint32_t zero = 0;
-  HIPCHK(c, hipMemcpy(c->ws.counters.as<uint32_t>() + kCntAdamStep, &zero, 4, hipMemcpyHostToDevice));
   return RGS_OK;
 }
 
@@ -978,8 +979,7 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
   HIPCHK(c, hipMemcpyAsync(sm.p, s->M.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(sv.p, s->V.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   uint32_t saved_step = 0;
-  HIPCHK(c, hipMemcpyAsync(&saved_step, c->ws.counters.as<uint32_t>() + kCntAdamStep, 4, hipMemcpyDeviceToHost,
-                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(&saved_step, s->step.p, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const rgs_raster_cfg orig = c->raster;
   rgs_raster_cfg best = orig;
@@ -1013,8 +1013,7 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
   HIPCHK(c, hipMemcpyAsync(s->P.p, sp.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s->M.p, sm.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s->V.p, sv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->ws.counters.as<uint32_t>() + kCntAdamStep, &saved_step, 4, hipMemcpyHostToDevice,
-                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(s->step.p, &saved_step, 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   release(sp);
   release(sm);

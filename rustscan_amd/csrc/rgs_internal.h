@@ -54,7 +54,7 @@ enum Counter : int {
   kCntQueueFwd = 3,     // persistent raster work counters
   kCntQueueBwd = 4,
   kCntStepValid = 5,    // 0 when the step must not update parameters
-  kCntAdamStep = 6,     // Adam step counter (all groups advance together)
+  kCntUnused6 = 6,
   kCntIsectMax = 7,     // max intersections of any forward in this step
   kCntSkipped = 8,      // steps whose update was skipped
   kCntNum = 16,
