@@ -103,6 +103,11 @@ def test_per_view_equals_batched(hip_ctx, setup):
 
 
 def test_graph_replay_tracks_eager(hip_ctx, setup):
+    """Each replay of the captured step computes the gradients an eager step
+    computes from the same parameters; after the first step (same Adam state)
+    the parameters agree too.  Later steps re-seed the eager store from the
+    graph's parameters: Adam's normalised updates would otherwise amplify the
+    last-bit differences of atomic summation order step by step."""
     import torch
 
     from rustscan_amd.rasterizer import DeviceSplats
@@ -112,17 +117,22 @@ def test_graph_replay_tracks_eager(hip_ctx, setup):
     a = DeviceSplats(hip_ctx, sc)
     b = DeviceSplats(hip_ctx, sc)
     for step in range(4):
+        if step > 0:
+            b.upload(a.download())
         tr.train_step(a, cams, tgt, batched=True, graph=True)
         hip_ctx.sync()
         torch.cuda.synchronize()
         tr.train_step(b, cams, tgt, batched=True, graph=False)
-        pa, pb = a.download(), b.download()
-        d = max(float(np.abs(pa.transforms - pb.transforms).max()), float(np.abs(pa.sh - pb.sh).max()),
-                float(np.abs(pa.raw_opac - pb.raw_opac).max()))
         ga, gb = a.grads(), b.grads()
-        ge = rel_l2(ga[0], gb[0])
-        say(f"  replay {step}: max param diff {d:.2e}, grad rel {ge:.2e}")
-        assert d <= 1e-5 and ge <= 1e-4
+        ge = max(rel_l2(ga[0], gb[0]), rel_l2(ga[1], gb[1]), rel_l2(ga[2], gb[2]))
+        say(f"  replay {step}: grad rel {ge:.2e}")
+        assert ge <= 1e-5
+        if step == 0:
+            pa, pb = a.download(), b.download()
+            d = max(float(np.abs(pa.transforms - pb.transforms).max()), float(np.abs(pa.sh - pb.sh).max()),
+                    float(np.abs(pa.raw_opac - pb.raw_opac).max()))
+            say(f"  first step max param diff {d:.2e}")
+            assert d <= 1e-5
     assert hip_ctx.skipped_steps() == 0
 
 
