@@ -146,21 +146,25 @@ def main() -> int:
     frags = {}
     counts = []
     if not args.no_instrument:
+        # timing pass: the production kernels, HIP events around each stage
         splats.upload(state)
         ctx.set_timing(True)
-        ctx.set_debug(count_fragments=True)
         ctx.timing(reset=True)
-        ctx.fragment_counts(reset=True)
         for _ in range(args.steps):
             trainer.train_step(splats, cams, targets, batched=True, graph=False)
         stages = ctx.timing(reset=True)
-        frags = ctx.fragment_counts(reset=True)
         counts = ctx.counts(len(cams))
         ctx.set_timing(False)
+        # counting pass: one step with the fragment-counting kernel variants
+        splats.upload(state)
+        ctx.set_debug(count_fragments=True)
+        ctx.fragment_counts(reset=True)
+        trainer.train_step(splats, cams, targets, batched=True, graph=False)
+        frags = ctx.fragment_counts(reset=True)
         ctx.set_debug()
         for k, (ms, calls) in stages.items():
             log(f"  stage {k:12s} {ms / max(calls, 1):8.3f} ms/launch  ({calls} launches)")
-        log(f"  fragments per step: {({k: v // args.steps for k, v in frags.items()})}")
+        log(f"  fragments per step: {frags}")
 
     roofline = None
     stage_summary = {}
@@ -170,7 +174,6 @@ def main() -> int:
         M = sum(c[1] for c in counts)
         nv = len(cams)
         k = 16
-        steps = args.steps
         per_launch = {}
         for name, (ms, calls) in stages.items():
             if calls:
@@ -185,8 +188,8 @@ def main() -> int:
             "emit": ("hbm", 52 * V + 8 * M),
             "tile_sort": ("hbm", 20 * M * sort_passes_tile),
             "tile_ranges": ("hbm", 12 * M),
-            "raster_fwd": ("mfma", 13 * frags.get("fwd_evaluated", 0) / steps + 9 * frags.get("fwd_blended", 0) / steps),
-            "raster_bwd": ("mfma", 13 * frags.get("bwd_evaluated", 0) / steps + 82 * frags.get("bwd_contributing", 0) / steps),
+            "raster_fwd": ("mfma", 13 * frags.get("fwd_evaluated", 0) + 9 * frags.get("fwd_blended", 0)),
+            "raster_bwd": ("mfma", 13 * frags.get("bwd_evaluated", 0) + 82 * frags.get("bwd_contributing", 0)),
             "project_bwd": ("hbm", (44 + 64) * n * nv + (11 + 3 * k + 7) * 4 * n),
             "adam": ("hbm", 28 * (11 + 3 * k) * n),
         }
@@ -283,7 +286,7 @@ def main() -> int:
         "stages": stage_summary,
         "counts": {"visible": int(sum(c[0] for c in counts)), "intersections": int(sum(c[1] for c in counts)),
                    "tiles": int(sum(c[2] for c in counts))} if counts else None,
-        "fragments_per_step": {k: v // args.steps for k, v in frags.items()} if frags else None,
+        "fragments_per_step": frags or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": dev_name,

@@ -223,8 +223,9 @@ def powi(base: float, e: int) -> float:
 
 
 def forward_loss_f64(splats64, sh_degree, cam, target, cov_blur=0.3, tile_width=16, flags=0, frozen_viewdir=None,
-                     frozen_comp=None, background=(0.0, 0.0, 0.0)):
-    """FP64 forward + L1 loss (mean over 3HW) for finite differences."""
+                     frozen_comp=None, background=(0.0, 0.0, 0.0), image_out=None):
+    """FP64 forward + L1 loss (mean over 3HW) for finite differences;
+    image_out ([H,W,4] float64, optional) receives the FP64 image."""
     tf, sh, op = [np.ascontiguousarray(a, np.float64) for a in splats64]
     n = op.shape[0]
     c = camera(cam)
@@ -232,10 +233,14 @@ def forward_loss_f64(splats64, sh_degree, cam, target, cov_blur=0.3, tile_width=
     fv = np.ascontiguousarray(frozen_viewdir, np.float64) if frozen_viewdir is not None else None
     fc = np.ascontiguousarray(frozen_comp, np.float64) if frozen_comp is not None else None
     tg = _f32(target)
+    if image_out is not None:
+        assert image_out.dtype == np.float64 and image_out.flags.c_contiguous
+        assert image_out.shape == (cam.height, cam.width, 4)
     return lib().orc_forward_loss_f64(n, sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data, ctypes.byref(c),
                                       bg.ctypes.data, float(cov_blur), int(tile_width), tg.ctypes.data, int(flags),
                                       fv.ctypes.data if fv is not None else None,
-                                      fc.ctypes.data if fc is not None else None, None)
+                                      fc.ctypes.data if fc is not None else None,
+                                      image_out.ctypes.data if image_out is not None else None)
 
 
 def projection_terms_f64(tf64, op64, cam, cov_blur=0.3):

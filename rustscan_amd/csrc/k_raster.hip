@@ -1,11 +1,14 @@
 // Per-tile alpha compositing, forward (rasterize.wgsl:21-99, K7) and backward
 // (rasterize_backwards.wgsl:24-169, K8).
 //
-// One workgroup composites one TWxTW tile, one thread per pixel.  The tile's
-// depth-sorted splats are staged through LDS in batches of BATCH.  Work items
-// are (view, tile) pairs numbered by global tile id: either one workgroup per
-// tile (grid launch) or a persistent grid pulling tiles from an atomic counter,
-// which is how uneven views share one launch.
+// One workgroup composites one TWxTW tile with one thread per pixel; each
+// 64-lane wave owns an 8x8 pixel block of the tile, so the lanes of a wave
+// are spatially compact (a splat's footprint covers fewer partially-active
+// waves than with row strips).  The tile's depth-sorted splats are staged
+// through LDS in batches of BATCH.  Work items are (view, tile) pairs
+// numbered by global tile id: either one workgroup per tile (grid launch) or
+// a persistent grid pulling tiles from an atomic counter, which is how uneven
+// views share one launch (MULTI: the tile's view is looked up per item).
 //
 // Forward changes versus the reference that do not change any result:
 //  * a workgroup stops loading batches once every pixel is done
@@ -17,16 +20,15 @@
 // K7's 0.5*(a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit (scaling by powers of
 // two commutes with rounding); the backward uses K8's own grouping.
 //
-// Backward: per splat the 64 lanes of a wave reduce their contributions
-// (12 values + a ballot count), lane 0 adds them into an LDS accumulator for
-// the batch, and after the batch one thread per splat adds the totals into the
-// per-instance accumulator with global atomics (one set per tile and splat
-// instead of one per pixel).
+// Backward: per splat the 64 lanes of a wave reduce their 12 contributions
+// with a transposed butterfly (permlane32 swap, permlane16 swap, then four
+// DPP row shifts), which leaves the 12 totals in lanes 15/31/47/63; those add
+// them into an LDS accumulator for the batch.  After the batch the totals go
+// to the per-instance accumulator with global atomics, one set per tile and
+// splat, issued so that the lanes of one instruction hit one 64-byte line.
 #include "rgs_device.h"
 
 namespace rgs {
-
-
 
 constexpr float kSkipMargin = 1e-3f;
 
@@ -43,28 +45,66 @@ __device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t* s_item,
   return *item < A.tile_count;
 }
 
+template <bool MULTI>
 __device__ __forceinline__ int view_of(const RasterArgs& A, uint32_t tile) {
   int v = A.v0;
-  for (int i = A.v0 + 1; i < A.v0 + A.nv; ++i)
-    if (tile >= A.views[i].tile_base) v = i;
+  if (MULTI)
+    for (int i = A.v0 + 1; i < A.v0 + A.nv; ++i)
+      if (tile >= A.views[i].tile_base) v = i;
   return v;
 }
 
-// Sum over the 64 lanes of a wave with DPP row shifts and row broadcasts;
-// the total is valid in lane 63.
+// Pixel of thread `tid` in a TWxTW tile: wave w covers the 8x8 block
+// (w % (TW/8), w / (TW/8)); lane l is pixel (l % 8, l / 8) of that block.
+template <int TW>
+__device__ __forceinline__ void pixel_in_tile(int tid, uint32_t* ox, uint32_t* oy) {
+  constexpr int BPR = TW / 8;
+  const int w = tid >> 6, l = tid & 63;
+  *ox = (uint32_t)((w % BPR) * 8 + (l & 7));
+  *oy = (uint32_t)((w / BPR) * 8 + (l >> 3));
+}
+
 #define RGS_DPP(v, ctrl, rm, bm) \
   __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, (v)), ctrl, rm, bm, true))
-__device__ __forceinline__ float wave_reduce_l63(float v) {
+
+// Sum of v over the 16 lanes of each row, valid in the row's lane 15.
+__device__ __forceinline__ float row_reduce_l15(float v) {
   v += RGS_DPP(v, 0x111, 0xf, 0xf);  // row_shr:1
   v += RGS_DPP(v, 0x112, 0xf, 0xf);  // row_shr:2
   v += RGS_DPP(v, 0x114, 0xf, 0xf);  // row_shr:4
-  v += RGS_DPP(v, 0x118, 0xf, 0xf);  // row_shr:8  -> lane 15 of each row holds the row sum
-  v += RGS_DPP(v, 0x142, 0xa, 0xf);  // row_bcast:15 into rows 1 and 3
-  v += RGS_DPP(v, 0x143, 0xc, 0xf);  // row_bcast:31 into rows 2 and 3
+  v += RGS_DPP(v, 0x118, 0xf, 0xf);  // row_shr:8
   return v;
 }
 
-template <int TW, int BATCH>
+// a <- (a.lanes[0:32] ++ b.lanes[0:32]) + (a.lanes[32:64] ++ b.lanes[32:64]):
+// lanes 0-31 hold half-sums of a, lanes 32-63 half-sums of b.
+__device__ __forceinline__ float fold32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                  false, false);
+  return __builtin_bit_cast(float, r[0]) + __builtin_bit_cast(float, r[1]);
+}
+
+// Same with 16-lane rows: rows 0 and 2 keep a's row pairs (0+1, 2+3), rows 1
+// and 3 b's.
+__device__ __forceinline__ float fold16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                  false, false);
+  return __builtin_bit_cast(float, r[0]) + __builtin_bit_cast(float, r[1]);
+}
+
+// Wave-wide sums of val[0..11].  On return, lane 16*r + 15 holds in out[a]
+// the total of val[a + 3*r] (a = 0..2, r = 0..3).  Every lane of the wave
+// must be active.
+__device__ __forceinline__ void wave_reduce12(const float val[12], float out[3]) {
+  float h[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) h[i] = fold32(val[i], val[i + 6]);  // rows 0,1: val[i]; rows 2,3: val[i+6]
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[a] = row_reduce_l15(fold16(h[a], h[a + 3]));
+  // after fold16, row 0: val[a], row 1: val[a+3], row 2: val[a+6], row 3: val[a+9]
+}
+
+template <int TW, int BATCH, bool MULTI, bool COUNT>
 __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
   constexpr int NT = TW * TW;
   __shared__ float4 s0[BATCH];  // x, y, a/2, b
@@ -74,14 +114,16 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
   __shared__ uint32_t sv[BATCH];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
+  uint32_t ox, oy;
+  pixel_in_tile<TW>(tid, &ox, &oy);
   uint32_t iter = 0, item;
   while (next_item(A, &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item;
-    const int v = view_of(A, tile);
+    const int v = view_of<MULTI>(A, tile);
     const ViewDev& V = A.views[v];
     const uint32_t local = tile - V.tile_base;
     const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
-    const uint32_t px = tx * TW + (tid % TW), py = ty * TW + (tid / TW);
+    const uint32_t px = tx * TW + ox, py = ty * TW + oy;
     const bool inside = px < V.W && py < V.H;
     const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
@@ -102,21 +144,28 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
       }
       __syncthreads();
       // Wave-uniform loop: every lane walks the batch until the whole wave is
-      // done; a finished lane only stops updating its pixel.
+      // done; a finished lane only stops updating its pixel.  The next
+      // splat's LDS record is fetched one iteration ahead.
+      float4 n0 = s0[0], n1 = s1[0];
+      float2 n2 = s2[0];
       for (uint32_t j = 0; j < cnt; ++j) {
         if (__all(done)) break;
-        const float4 a0 = s0[j];
-        const float4 a1 = s1[j];
-        const float2 a2 = s2[j];
+        const float4 a0 = n0, a1 = n1;
+        const float2 a2 = n2;
+        if (j + 1 < cnt) {
+          n0 = s0[j + 1];
+          n1 = s1[j + 1];
+          n2 = s2[j + 1];
+        }
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
-        n_eval += done ? 0u : 1u;
+        if (COUNT) n_eval += done ? 0u : 1u;
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
         const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core(-sigma));
         const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
         if (acc) {
-          ++n_blend;
+          if (COUNT) ++n_blend;
           const float vis = alpha * T;
           cr = cr + a1.z * vis;
           cg = cg + a1.w * vis;
@@ -145,7 +194,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
       const float s = wave_sum(l1);
       if ((tid & 63) == 0 && s != 0.0f) atomicAdd(&A.loss[v], s);
     }
-    if (A.frag) {
+    if (COUNT) {
       unsigned long long e = n_eval, bl = n_blend;
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) {
@@ -160,26 +209,32 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
   }
 }
 
-template <int TW, int BATCH, bool REFINE>
+template <int TW, int BATCH, bool REFINE, bool MULTI, bool COUNT>
 __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
   constexpr int NT = TW * TW;
-  constexpr int NA = 13;  // accumulated values per splat
+  // Accumulator slots per splat; the layout is that of the per-instance
+  // accumulator: 0-1 v_xy, 2-4 v_conic, 5-7 v_rgb, 8 opacity term, 9 refine,
+  // 10-11 |v_xy|, 12 coverage count.
+  constexpr int NA = 13;
+  constexpr int SA = 16;  // LDS stride per splat
   __shared__ float4 s0[BATCH];  // x, y, a/2, b
   __shared__ float4 s1[BATCH];  // c/2, opacity, r, g (unclamped)
   __shared__ float2 s2[BATCH];  // b (unclamped), exp-skip bound
   __shared__ uint32_t si[BATCH];
-  __shared__ float sacc[BATCH * NA];
+  __shared__ float sacc[BATCH * SA];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  uint32_t ox, oy;
+  pixel_in_tile<TW>(tid, &ox, &oy);
   uint32_t iter = 0, item;
   while (next_item(A, &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item;
-    const int v = view_of(A, tile);
+    const int v = view_of<MULTI>(A, tile);
     const ViewDev& V = A.views[v];
     const uint32_t local = tile - V.tile_base;
     const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
-    const uint32_t px = tx * TW + (tid % TW), py = ty * TW + (tid / TW);
+    const uint32_t px = tx * TW + ox, py = ty * TW + oy;
     const bool inside = px < V.W && py < V.H;
     const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
@@ -224,16 +279,22 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         s2[j] = make_float2(q.p2.x, q.p2.y + kSkipMargin);
         si[j] = inst;
       }
-      for (uint32_t k = tid; k < cnt * NA; k += NT) sacc[k] = 0.0f;
+      for (uint32_t k = tid; k < cnt * SA; k += NT) sacc[k] = 0.0f;
       __syncthreads();
+      float4 n0 = s0[0], n1 = s1[0];
+      float2 n2 = s2[0];
       for (uint32_t j = 0; j < cnt; ++j) {
         if (__all(done)) break;
-        const float4 a0 = s0[j];
-        const float4 a1 = s1[j];
-        const float2 a2 = s2[j];
+        const float4 a0 = n0, a1 = n1;
+        const float2 a2 = n2;
+        if (j + 1 < cnt) {
+          n0 = s0[j + 1];
+          n1 = s1[j + 1];
+          n2 = s2[j + 1];
+        }
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
-        n_eval += done ? 0u : 1u;
+        if (COUNT) n_eval += done ? 0u : 1u;
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
         const float gauss = rgs_exp_core(-sigma);
@@ -249,7 +310,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) val[k] = 0.0f;
         if (contrib) {
-          ++n_contrib;
+          if (COUNT) ++n_contrib;
           const float col[3] = {a1.z, a1.w, a2.x};
           const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
           const float vis = alpha * T;
@@ -258,7 +319,9 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
             val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
             po[c] = po[c] + vis * cl[c];
           }
-          const float ra = 1.0f / (1.0f - alpha);
+          // 1/(1-alpha) with the hardware reciprocal (1 ulp); gradients are
+          // compared within a relative tolerance, not bit for bit.
+          const float ra = __builtin_amdgcn_rcpf(1.0f - alpha);
           float tv[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
@@ -282,32 +345,35 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
             val[10] = fabsf(vx);
             val[11] = fabsf(vy);
           }
+          if (!REFINE) val[9] = 1.0f;  // coverage count rides in the refine slot
           T = nT;
         }
+        float tot[3];
+        wave_reduce12(val, tot);
+        if ((lane & 15) == 15) {
+          const int r = lane >> 4;
+          float* dst = sacc + j * SA;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          if (k == 9 && !REFINE) continue;
-          val[k] = wave_reduce_l63(val[k]);
-        }
-        if (lane == 63) {
-          float* dst = sacc + j * NA;
-#pragma unroll
-          for (int k = 0; k < 12; ++k)
-            if (val[k] != 0.0f) atomicAdd(dst + k, val[k]);
-          atomicAdd(dst + 12, (float)__popcll(m));
+          for (int a = 0; a < 3; ++a) {
+            const int k = a + 3 * r;
+            const int slot = (!REFINE && k == 9) ? 12 : k;
+            atomicAdd(dst + slot, tot[a]);
+          }
+          if (REFINE && lane == 63) atomicAdd(dst + 12, (float)__popcll(m));
         }
       }
       __syncthreads();
-      for (uint32_t j = tid; j < cnt; j += NT) {
-        const float* src = sacc + j * NA;
-        if (src[12] == 0.0f) continue;
-        float* dst = A.acc + (size_t)si[j] * kAcc;
-#pragma unroll
-        for (int k = 0; k < NA; ++k)
-          if (src[k] != 0.0f) atomicAdd(dst + k, src[k]);
+      // Flush: 16 consecutive threads per splat, one float each, so each
+      // atomic instruction covers whole 64-byte accumulator lines.
+      for (uint32_t e = tid; e < cnt * SA; e += NT) {
+        const uint32_t j = e / SA, k = e % SA;
+        if (k >= (uint32_t)NA) continue;
+        if (sacc[j * SA + 12] == 0.0f) continue;
+        const float x = sacc[e];
+        if (x != 0.0f) atomicAdd(A.acc + (size_t)si[j] * kAcc + k, x);
       }
     }
-    if (A.frag) {
+    if (COUNT) {
       unsigned long long e = n_eval, cb2 = n_contrib;
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) {
@@ -323,22 +389,34 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
 }
 
 // ------------------------------------------------------------------ launch
+// Variants: the per-view path (one view per launch) skips the view lookup;
+// fragment counting and the refine statistic are instrumentation variants
+// built only for the multi-view form.
 template <int TW, int BATCH>
 static void fwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid) {
-  k_raster_fwd<TW, BATCH><<<grid, TW * TW, 0, s>>>(A);
+  if (A.frag)
+    k_raster_fwd<TW, BATCH, true, true><<<grid, TW * TW, 0, s>>>(A);
+  else if (A.nv > 1)
+    k_raster_fwd<TW, BATCH, true, false><<<grid, TW * TW, 0, s>>>(A);
+  else
+    k_raster_fwd<TW, BATCH, false, false><<<grid, TW * TW, 0, s>>>(A);
 }
 template <int TW, int BATCH>
 static void bwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid, bool refine) {
   if (refine)
-    k_raster_bwd<TW, BATCH, true><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, true, true, false><<<grid, TW * TW, 0, s>>>(A);
+  else if (A.frag)
+    k_raster_bwd<TW, BATCH, false, true, true><<<grid, TW * TW, 0, s>>>(A);
+  else if (A.nv > 1)
+    k_raster_bwd<TW, BATCH, false, true, false><<<grid, TW * TW, 0, s>>>(A);
   else
-    k_raster_bwd<TW, BATCH, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, false, false, false><<<grid, TW * TW, 0, s>>>(A);
 }
 
-#define RGS_DISPATCH(TWV, BV, CALL)                     \
-  if (tw == TWV && batch == BV) {                       \
-    CALL;                                               \
-    return 0;                                           \
+#define RGS_DISPATCH(TWV, BV, CALL) \
+  if (tw == TWV && batch == BV) {   \
+    CALL;                           \
+    return 0;                       \
   }
 
 int launch_raster_fwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid) {
@@ -371,12 +449,14 @@ int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uin
 int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus) {
   int per_cu = 0;
   const int threads = tw * tw;
-#define RGS_OCC(TWV, BV)                                                                                     \
-  if (tw == TWV && batch == BV) {                                                                            \
-    if (bwd)                                                                                                 \
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_bwd<TWV, BV, false>, threads, 0); \
-    else                                                                                                     \
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_fwd<TWV, BV>, threads, 0);  \
+#define RGS_OCC(TWV, BV)                                                                                \
+  if (tw == TWV && batch == BV) {                                                                       \
+    if (bwd)                                                                                            \
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_bwd<TWV, BV, false, true, false>, \
+                                                   threads, 0);                                         \
+    else                                                                                                \
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_raster_fwd<TWV, BV, true, false>, \
+                                                   threads, 0);                                         \
   }
   RGS_OCC(8, 128) RGS_OCC(8, 256) RGS_OCC(8, 512)
   RGS_OCC(16, 128) RGS_OCC(16, 256) RGS_OCC(16, 512)
