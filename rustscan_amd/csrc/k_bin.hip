@@ -6,8 +6,10 @@
 // 8-bit digits; each pass is
 //   k_radix_hist    per-block digit histograms          hist[digit][block]
 //   k_radix_spine   one block per digit: exclusive scan of its row over blocks
-//   k_radix_scatter stable scatter; a block's items are ranked per 64-lane
-//                   wave with ballots (bitwise match of the digit), in index order.
+//   k_radix_scatter per 4096-item tile: each wave ranks its 1024 contiguous
+//                   items with ballots (bitwise match of the digit), the tile
+//                   is sorted by digit in LDS, and written out so that
+//                   consecutive threads store consecutive slots of a bucket.
 // Every kernel reads the item count from device memory (clamped to capacity),
 // so the sequence is graph-capturable with no host readback.
 //
@@ -19,15 +21,18 @@
 namespace rgs {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortTile = 1024;  // items per block iteration (4 waves x 4 slots x 64 lanes)
+constexpr int kSortTile = 1024;  // scan: items per block iteration (256 threads x 4)
+constexpr int kRadixRows = 16;   // radix: items per lane per tile
+constexpr int kRadixTile = kSortThreads * kRadixRows;  // 4096 items; each wave owns 1024 contiguous
 
 __device__ __forceinline__ uint32_t dev_count(const uint32_t* n_dev, uint32_t n_host, uint32_t cap) {
   const uint32_t n = n_dev ? *n_dev : n_host;
   return n < cap ? n : cap;
 }
 
+template <int TILE = kSortTile>
 __device__ __forceinline__ void block_tiles(uint32_t n, uint32_t* t0, uint32_t* t1) {
-  const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+  const uint32_t tiles = (n + TILE - 1) / TILE;
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const uint32_t a = blockIdx.x * per;
   *t0 = a < tiles ? a : tiles;
@@ -68,9 +73,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
   __syncthreads();
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
-  block_tiles(n, &t0, &t1);
-  const uint32_t end = t1 * kSortTile < n ? t1 * kSortTile : n;
-  for (uint32_t i = t0 * kSortTile + threadIdx.x; i < end; i += kSortThreads)
+  block_tiles<kRadixTile>(n, &t0, &t1);
+  const uint32_t end = t1 * kRadixTile < n ? t1 * kRadixTile : n;
+  for (uint32_t i = t0 * kRadixTile + threadIdx.x; i < end; i += kSortThreads)
     atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t d = threadIdx.x;
@@ -106,8 +111,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev, uint32_t n_host, uint32_t cap, int shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot) {
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t wh[4][256];
+  __shared__ uint32_t base[256];     // next global slot of each digit for this block
+  __shared__ uint32_t gbase[256];    // global slot of local position 0 of each digit's run in the tile
+  __shared__ uint32_t wh[4][256];    // per-wave digit counts, then per-wave local bases
+  __shared__ uint32_t sk[kRadixTile], sv[kRadixTile];
   __shared__ uint32_t tmp[8];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   {
@@ -117,53 +124,73 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   }
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
-  block_tiles(n, &t0, &t1);
+  block_tiles<kRadixTile>(n, &t0, &t1);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   for (uint32_t t = t0; t < t1; ++t) {
     for (int i = threadIdx.x; i < 1024; i += kSortThreads) (&wh[0][0])[i] = 0;
     __syncthreads();
-    uint32_t key[4], val[4], dig[4], loc[4];
-    bool ok[4];
+    // 1. each wave ranks its 1024 contiguous items row by row (64 per row),
+    //    in index order, with a running count per digit
+    uint32_t key[kRadixRows], val[kRadixRows], loc[kRadixRows];
+    const uint32_t wbase = t * kRadixTile + (uint32_t)w * (64 * kRadixRows);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const uint32_t idx = t * kSortTile + w * 256 + s * 64 + lane;
-      ok[s] = idx < n;
-      key[s] = ok[s] ? kin[idx] : 0u;
-      val[s] = ok[s] ? (vin ? vin[idx] : idx) : 0u;
-      dig[s] = (key[s] >> shift) & 255u;
-      uint64_t peers = __ballot(ok[s]);
+    for (int r = 0; r < kRadixRows; ++r) {
+      const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
+      const bool ok = idx < n;
+      key[r] = ok ? kin[idx] : 0u;
+      val[r] = ok ? (vin ? vin[idx] : idx) : 0u;
+      const uint32_t d = (key[r] >> shift) & 255u;
+      uint64_t peers = __ballot(ok);
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const bool bit = (dig[s] >> b) & 1u;
+        const bool bit = (d >> b) & 1u;
         const uint64_t m = __ballot(bit);
         peers &= bit ? m : ~m;
       }
       const uint32_t rank = (uint32_t)__popcll(peers & lt);
       uint32_t old = 0;
-      if (ok[s]) old = wh[w][dig[s]];
-      if (ok[s] && rank == 0) wh[w][dig[s]] = old + (uint32_t)__popcll(peers);
-      loc[s] = old + rank;
+      if (ok) old = wh[w][d];
+      if (ok && rank == 0) wh[w][d] = old + (uint32_t)__popcll(peers);
+      loc[r] = old + rank;
     }
     __syncthreads();
+    // 2. per digit: local start in the sorted tile (scan over digits), the
+    //    base of each wave's run, and the global slot of the digit's run
     {
       const uint32_t d = threadIdx.x;
-      uint32_t run = base[d];
-#pragma unroll
-      for (int ww = 0; ww < 4; ++ww) {
-        const uint32_t c = wh[ww][d];
-        wh[ww][d] = run;
-        run += c;
-      }
-      base[d] = run;
+      const uint32_t c0 = wh[0][d], c1 = wh[1][d], c2 = wh[2][d], c3 = wh[3][d];
+      const uint32_t cnt = c0 + c1 + c2 + c3;
+      uint32_t total;
+      const uint32_t start = block_excl_scan256(cnt, tmp, &total);
+      wh[0][d] = start;
+      wh[1][d] = start + c0;
+      wh[2][d] = start + c0 + c1;
+      wh[3][d] = start + c0 + c1 + c2;
+      gbase[d] = base[d] - start;
+      base[d] += cnt;
     }
     __syncthreads();
+    // 3. sort the tile by digit in LDS (stable: wave order, then rank)
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      if (ok[s]) {
-        const uint32_t dst = wh[w][dig[s]] + loc[s];
-        kout[dst] = key[s];
-        vout[dst] = val[s];
+    for (int r = 0; r < kRadixRows; ++r) {
+      const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
+      if (idx < n) {
+        const uint32_t p = wh[w][(key[r] >> shift) & 255u] + loc[r];
+        sk[p] = key[r];
+        sv[p] = val[r];
       }
+    }
+    __syncthreads();
+    // 4. write out: thread i stores local position i, so each digit's run
+    //    goes to consecutive global slots from consecutive threads
+    const uint32_t tn = min((uint32_t)kRadixTile, n - t * kRadixTile);
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < tn; i += kSortThreads) {
+      const uint32_t k = sk[i];
+      const uint32_t dst = gbase[(k >> shift) & 255u] + i;
+      kout[dst] = k;
+      vout[dst] = sv[i];
+    }
     __syncthreads();
   }
 }
@@ -247,13 +274,17 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_down(const uint32_t* __re
 // ------------------------------------------------------------- emission
 // map_gaussian_to_intersects.wgsl:15-49: for each visible instance in depth
 // order, write (global tile id, instance) for every tile of its bbox that
-// passes the tile test, at offs[i] + k.  The 256 instances of a workgroup own
-// one contiguous output range, which is assembled in LDS and written out
-// coalesced (ranges too large for LDS are written directly).  Bboxes of at most
-// 64 tiles reuse the pass mask of the projection pass instead of re-testing.
+// passes the tile test, at offs[i] + k, in row-major bbox order.
+//  * Bboxes of at most 64 tiles carry their pass mask from the projection
+//    pass; the 256 instances of a workgroup own one contiguous output range,
+//    which is assembled in LDS in windows of kEmitStage slots and written out
+//    coalesced.
+//  * Larger bboxes are tested by the whole wave, 64 tiles per step, and their
+//    (long, contiguous) runs go straight to memory.
 // Writes past `cap` are dropped and flagged; the step is then marked invalid
 // so no parameter is updated.
 constexpr uint32_t kEmitStage = 4096;
+constexpr uint32_t kEmptySlot = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj, const uint32_t* __restrict__ order,
                                               const uint32_t* __restrict__ offs, const uint32_t* __restrict__ cnt,
@@ -263,7 +294,6 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
                                               uint32_t* __restrict__ ovals) {
   __shared__ uint32_t sk[kEmitStage], sv[kEmitStage];
   __shared__ uint32_t s_base, s_total;
-  __shared__ int s_over;
   const uint32_t V = min(counters[kCntVisible], cap_inst);
   const uint32_t i0 = blockIdx.x * blockDim.x;
   if (i0 >= V) return;  // whole workgroup
@@ -271,55 +301,108 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
     const uint32_t last = min(i0 + blockDim.x, V) - 1;
     s_base = offs[i0];
     s_total = offs[last] + cnt[order[last]] - offs[i0];
-    s_over = 0;
   }
-  __syncthreads();
-  const uint32_t base = s_base, total = s_total;
-  const bool staged = total <= kEmitStage;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
   const uint32_t i = i0 + threadIdx.x;
-  bool over = false;
-  if (i < V) {
-    const uint32_t inst = order[i];
-    const uint32_t v = inst / n_per_view;
-    const ViewDev& VD = views[v];
+  const bool live = i < V;
+  uint32_t inst = 0, pos = 0, bb[4] = {0u, 0u, 0u, 0u}, tbase = 0, tx_stride = 0;
+  float mx = 0.f, my = 0.f, a = 0.f, b = 0.f, c = 0.f, thr = 0.f;
+  uint64_t mask = 0;
+  bool big = false;
+  if (live) {
+    inst = order[i];
+    const ViewDev& VD = views[inst / n_per_view];
+    tbase = VD.tile_base;
+    tx_stride = VD.tiles_x;
     const ProjSplat s = proj[inst];
-    uint32_t bb[4];
     unpack_u16x2(s.p2.z, &bb[0], &bb[1]);
     unpack_u16x2(s.p2.w, &bb[2], &bb[3]);
-    const float a = 2.0f * s.p0.z, b = s.p0.w, c = 2.0f * s.p1.x, thr = s.p2.y;
-    const uint32_t nb = (bb[3] - bb[1]) * (bb[2] - bb[0]);
-    const uint64_t mask = (uint64_t)__float_as_uint(s.p3.x) | ((uint64_t)__float_as_uint(s.p3.y) << 32);
-    uint32_t pos = offs[i];
-    uint32_t k = 0;
-    for (uint32_t ty = bb[1]; ty < bb[3]; ++ty)
-      for (uint32_t tx = bb[0]; tx < bb[2]; ++tx, ++k) {
-        const bool hit = nb <= 64 ? ((mask >> k) & 1ull) != 0
-                                  : tile_contributes(tx, ty, tw, s.p0.x, s.p0.y, a, b, c, thr);
-        if (!hit) continue;
-        const uint32_t key = VD.tile_base + tx + ty * VD.tiles_x;
-        if (staged) {
-          sk[pos - base] = key;
-          sv[pos - base] = inst;
-        } else if (pos < cap) {
-          okeys[pos] = key;
-          ovals[pos] = inst;
+    mx = s.p0.x;
+    my = s.p0.y;
+    a = 2.0f * s.p0.z;
+    b = s.p0.w;
+    c = 2.0f * s.p1.x;
+    thr = s.p2.y;
+    mask = (uint64_t)__float_as_uint(s.p3.x) | ((uint64_t)__float_as_uint(s.p3.y) << 32);
+    big = (bb[3] - bb[1]) * (bb[2] - bb[0]) > 64u;
+    pos = offs[i];
+  }
+  bool over = false;
+  // large bboxes: one wave per instance, direct coalesced runs
+  uint64_t bigm = __ballot(big);
+  while (bigm) {
+    const int j = __builtin_ctzll(bigm);
+    bigm &= bigm - 1;
+    uint32_t jb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) jb[e] = (uint32_t)__shfl((int)bb[e], j, 64);
+    const float jx = __shfl(mx, j, 64), jy = __shfl(my, j, 64), ja = __shfl(a, j, 64), jbb = __shfl(b, j, 64),
+                jc = __shfl(c, j, 64), jt = __shfl(thr, j, 64);
+    const uint32_t jinst = (uint32_t)__shfl((int)inst, j, 64), jpos = (uint32_t)__shfl((int)pos, j, 64);
+    const uint32_t jtb = (uint32_t)__shfl((int)tbase, j, 64), jts = (uint32_t)__shfl((int)tx_stride, j, 64);
+    const uint32_t bw = jb[2] - jb[0], nb = (jb[3] - jb[1]) * bw;
+    const float inv = 1.0f / (float)bw;
+    uint32_t run = jpos;
+    for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      bool hit = false;
+      uint32_t row = 0, col = 0;
+      if (k < nb) {
+        bbox_rc(k, bw, inv, &row, &col);
+        hit = tile_contributes(jb[0] + col, jb[1] + row, tw, jx, jy, ja, jbb, jc, jt);
+      }
+      const uint64_t m = __ballot(hit);
+      if (hit) {
+        const uint32_t p = run + (uint32_t)__popcll(m & lt);
+        if (p < cap) {
+          okeys[p] = jtb + (jb[0] + col) + (jb[1] + row) * jts;
+          ovals[p] = jinst;
         } else {
           over = true;
         }
-        ++pos;
       }
+      run += (uint32_t)__popcll(m);
+    }
   }
-  if (staged) {
+  // small bboxes: stage the block's range in windows, write out coalesced
+  __syncthreads();
+  const uint32_t base = s_base, total = s_total;
+  const uint32_t bw = bb[2] - bb[0];
+  const float inv = 1.0f / (float)(bw ? bw : 1u);
+  for (uint32_t w0 = 0; w0 < total; w0 += kEmitStage) {
+    for (uint32_t e = threadIdx.x; e < kEmitStage; e += blockDim.x) sv[e] = kEmptySlot;
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-      const uint32_t p = base + e;
+    if (live && !big) {
+      const uint32_t lo = base + w0, hi = lo + kEmitStage;
+      uint64_t m = mask;
+      uint32_t p = pos;
+      while (m && p < hi) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        if (p >= lo) {
+          uint32_t row, col;
+          bbox_rc(k, bw, inv, &row, &col);
+          sk[p - lo] = tbase + (bb[0] + col) + (bb[1] + row) * tx_stride;
+          sv[p - lo] = inst;
+        }
+        ++p;
+      }
+    }
+    __syncthreads();
+    const uint32_t wn = min(kEmitStage, total - w0);
+    for (uint32_t e = threadIdx.x; e < wn; e += blockDim.x) {
+      const uint32_t val = sv[e];
+      if (val == kEmptySlot) continue;
+      const uint32_t p = base + w0 + e;
       if (p < cap) {
         okeys[p] = sk[e];
-        ovals[p] = sv[e];
+        ovals[p] = val;
       } else {
         over = true;
       }
     }
+    __syncthreads();
   }
   if (over) {
     counters[kCntOverflow] = 1u;
@@ -373,8 +456,9 @@ __global__ void k_step_end(uint32_t* __restrict__ counters) {
 }
 
 // ------------------------------------------------------------------ host side
+template <int TILE = kSortTile>
 static uint32_t grid_for(uint64_t cap) {
-  uint64_t tiles = (cap + kSortTile - 1) / kSortTile;
+  uint64_t tiles = (cap + TILE - 1) / TILE;
   if (tiles < 1) tiles = 1;
   return (uint32_t)(tiles < 2048 ? tiles : 2048);
 }
@@ -387,13 +471,13 @@ static uint32_t grid_for(uint64_t cap) {
 // Returns 1 if the result is in (kb, vb), 2 if in (kc, vc), 0 if key_bits <= 0
 // (nothing done; the input order is already sorted).
 // scratch: 256 * G + 256 uint32 with G = radix_grid(cap).
-uint32_t radix_grid(uint64_t cap) { return grid_for(cap); }
+uint32_t radix_grid(uint64_t cap) { return grid_for<kRadixTile>(cap); }
 
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
                      uint32_t* scratch) {
   if (key_bits <= 0) return 0;
-  const uint32_t G = grid_for(cap);
+  const uint32_t G = grid_for<kRadixTile>(cap);
   uint32_t* hist = scratch;
   uint32_t* tot = scratch + 256 * (size_t)G;
   const int passes = (key_bits + 7) / 8;
@@ -415,7 +499,7 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
 
 // Exclusive (or inclusive) scan of in[gather[i]] (gather may be null) for
 // i < n; the grand total is written to *total_dev when non-null.
-// scratch: radix_grid(cap) uint32.
+// scratch: grid_for(cap) <= 4 * radix_grid(cap) uint32.
 void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const uint32_t* n_dev, uint32_t n_host,
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev,
               uint32_t* scratch) {

@@ -180,82 +180,102 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
                                                      uint32_t* __restrict__ cnt, uint32_t* __restrict__ counters) {
   constexpr int NK = (DEG + 1) * (DEG + 1);
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = gid < n;
+  const uint32_t gl = live ? gid : 0u;  // lanes past n run the view loop with pass = false
   uint32_t nvis = 0;
-  if (gid < n) {
-    const float mean[3] = {P[gid], P[npad + gid], P[2 * npad + gid]};
-    const float q0 = P[3 * npad + gid], q1 = P[4 * npad + gid], q2 = P[5 * npad + gid], q3 = P[6 * npad + gid];
-    const float sc[3] = {rgs_exp(P[7 * npad + gid]), rgs_exp(P[8 * npad + gid]), rgs_exp(P[9 * npad + gid])};
-    const float sig = rgs_sigmoid(P[10 * npad + gid]);
-    float sh[NK * 3];
+  const float mean[3] = {P[gl], P[npad + gl], P[2 * npad + gl]};
+  const float q0 = P[3 * npad + gl], q1 = P[4 * npad + gl], q2 = P[5 * npad + gl], q3 = P[6 * npad + gl];
+  const float sc[3] = {rgs_exp(P[7 * npad + gl]), rgs_exp(P[8 * npad + gl]), rgs_exp(P[9 * npad + gl])};
+  const float sig = rgs_sigmoid(P[10 * npad + gl]);
+  float sh[NK * 3];
 #pragma unroll
-    for (int i = 0; i < NK * 3; ++i) sh[i] = P[(size_t)(11 + i) * npad + gid];
-    const float qn = ((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3;
-    const bool qok = !(qn < RGS_F(1e-6));
-    const float is = 1.0f / rgs_sqrt(qn);
-    Geom g;
-    make_geom(q0 * is, q1 * is, q2 * is, q3 * is, sc, g);
-    for (int v = v0; v < v0 + nv; ++v) {
-      const ViewDev& V = views[v];
-      const size_t inst = (size_t)v * n + gid;
-      const float mcx = ((V.vm[0] * mean[0] + V.vm[4] * mean[1]) + V.vm[8] * mean[2]) + V.vm[12];
-      const float mcy = ((V.vm[1] * mean[0] + V.vm[5] * mean[1]) + V.vm[9] * mean[2]) + V.vm[13];
-      const float mcz = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
-      bool pass = qok && !(mcz < RGS_F(0.01) || mcz > RGS_F(1e10));
-      uint32_t count = 0;
+  for (int i = 0; i < NK * 3; ++i) sh[i] = P[(size_t)(11 + i) * npad + gl];
+  const float qn = ((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3;
+  const bool qok = live && !(qn < RGS_F(1e-6));
+  const float is = 1.0f / rgs_sqrt(qn);
+  Geom g;
+  make_geom(q0 * is, q1 * is, q2 * is, q3 * is, sc, g);
+  for (int v = v0; v < v0 + nv; ++v) {
+    const ViewDev& V = views[v];
+    const size_t inst = (size_t)v * n + gid;
+    const float mcx = ((V.vm[0] * mean[0] + V.vm[4] * mean[1]) + V.vm[8] * mean[2]) + V.vm[12];
+    const float mcy = ((V.vm[1] * mean[0] + V.vm[5] * mean[1]) + V.vm[9] * mean[2]) + V.vm[13];
+    const float mcz = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
+    bool pass = qok && !(mcz < RGS_F(0.01) || mcz > RGS_F(1e10));
+    uint32_t count = 0;
+    bool big = false;
+    float m2x = 0.f, m2y = 0.f, a = 0.f, b = 0.f, c = 0.f, thr = 0.f;
+    uint32_t bb[4] = {0u, 0u, 0u, 0u};
+    if (pass) {
+      Proj2 pc;
+      project_cov(g, V, mcx, mcy, mcz, pc);
+      // compensate_cov2d (helpers.wgsl:170-183)
+      M2 c2 = pc.cov2d;
+      const float det_raw = fmaxf_(det2(c2), 0.0f);
+      c2.c00 = c2.c00 + blur;
+      c2.c11 = c2.c11 + blur;
+      const float det_end = fmaxf_(det2(c2), RGS_F(1e-12));
+      const float comp = det_raw <= 0.0f ? 0.0f : rgs_sqrt(det_raw / det_end);
+      const float opacity = sig * comp;
+      pass = !(opacity < RGS_F(1.0 / 255.0));
       if (pass) {
-        Proj2 pc;
-        project_cov(g, V, mcx, mcy, mcz, pc);
-        // compensate_cov2d (helpers.wgsl:170-183)
-        M2 c2 = pc.cov2d;
-        const float det_raw = fmaxf_(det2(c2), 0.0f);
-        c2.c00 = c2.c00 + blur;
-        c2.c11 = c2.c11 + blur;
-        const float det_end = fmaxf_(det2(c2), RGS_F(1e-12));
-        const float comp = det_raw <= 0.0f ? 0.0f : rgs_sqrt(det_raw / det_end);
-        const float opacity = sig * comp;
-        pass = !(opacity < RGS_F(1.0 / 255.0));
+        const float rzm = 1.0f / mcz;
+        m2x = V.fx * mcx * rzm + V.cx;
+        m2y = V.fy * mcy * rzm + V.cy;
+        const M2 con = inverse2x2(c2);
+        thr = rgs_log(opacity * 255.0f);
+        float ex, ey;
+        bbox_extent(c2.c00, c2.c11, thr, &ex, &ey);
+        if (ex < 0.0f || ey < 0.0f) pass = false;
+        if (m2x + ex <= 0.0f || m2x - ex >= (float)V.W || m2y + ey <= 0.0f || m2y - ey >= (float)V.H)
+          pass = false;
         if (pass) {
-          const float rzm = 1.0f / mcz;
-          const float m2x = V.fx * mcx * rzm + V.cx;
-          const float m2y = V.fy * mcy * rzm + V.cy;
-          const M2 con = inverse2x2(c2);
-          const float thr = rgs_log(opacity * 255.0f);
-          float ex, ey;
-          bbox_extent(c2.c00, c2.c11, thr, &ex, &ey);
-          if (ex < 0.0f || ey < 0.0f) pass = false;
-          if (m2x + ex <= 0.0f || m2x - ex >= (float)V.W || m2y + ey <= 0.0f || m2y - ey >= (float)V.H)
-            pass = false;
-          if (pass) {
-            const float a = con.c00, b = con.c01, c = con.c11;
-            // K5 rule: extent from the inverse of the conic
-            const M2 cv = inverse2x2(M2{a, b, b, c});
-            float ex5, ey5;
-            bbox_extent(cv.c00, cv.c11, thr, &ex5, &ey5);
-            uint32_t bb[4];
-            tile_bbox(m2x, m2y, ex5, ey5, V.tiles_x, V.tiles_y, tw, bb);
-            uint64_t mask;
-            count = count_tiles(bb, tw, m2x, m2y, a, b, c, thr, &mask);
-            // colour
-            const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
-            const float dd = (dx * dx + dy * dy) + dz * dz;
-            const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
-            float col[3];
-            sh_color<DEG>(sh, dx * rs, dy * rs, dz * rs, col);
-            ProjSplat ps;
-            ps.p0 = make_float4(m2x, m2y, 0.5f * a, b);
-            ps.p1 = make_float4(0.5f * c, opacity, col[0] + 0.5f, col[1] + 0.5f);
-            ps.p2 = make_float4(col[2] + 0.5f, thr, pack_u16x2(bb[0], bb[1]), pack_u16x2(bb[2], bb[3]));
-            ps.p3 = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.0f, 0.0f);
-            proj[inst] = ps;
-            keys[inst] = __float_as_uint(mcz);
-          }
+          a = con.c00;
+          b = con.c01;
+          c = con.c11;
+          // K5 rule: extent from the inverse of the conic
+          const M2 cv = inverse2x2(M2{a, b, b, c});
+          float ex5, ey5;
+          bbox_extent(cv.c00, cv.c11, thr, &ex5, &ey5);
+          tile_bbox(m2x, m2y, ex5, ey5, V.tiles_x, V.tiles_y, tw, bb);
+          const uint32_t nb = (bb[3] - bb[1]) * (bb[2] - bb[0]);
+          uint64_t mask = 0;
+          big = nb > 64u;
+          if (!big) count = count_tiles(bb, tw, m2x, m2y, a, b, c, thr, &mask);
+          // colour
+          const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
+          const float dd = (dx * dx + dy * dy) + dz * dz;
+          const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
+          float col[3];
+          sh_color<DEG>(sh, dx * rs, dy * rs, dz * rs, col);
+          ProjSplat ps;
+          ps.p0 = make_float4(m2x, m2y, 0.5f * a, b);
+          ps.p1 = make_float4(0.5f * c, opacity, col[0] + 0.5f, col[1] + 0.5f);
+          ps.p2 = make_float4(col[2] + 0.5f, thr, pack_u16x2(bb[0], bb[1]), pack_u16x2(bb[2], bb[3]));
+          ps.p3 = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.0f, 0.0f);
+          proj[inst] = ps;
+          keys[inst] = __float_as_uint(mcz);
         }
       }
+    }
+    // bboxes of more than 64 tiles: the wave counts them together
+    uint64_t bigm = __ballot(big);
+    while (bigm) {
+      const int j = __builtin_ctzll(bigm);
+      bigm &= bigm - 1;
+      uint32_t jb[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) jb[e] = (uint32_t)__shfl((int)bb[e], j, 64);
+      const uint32_t tot = wave_count_tiles(jb, tw, __shfl(m2x, j, 64), __shfl(m2y, j, 64), __shfl(a, j, 64),
+                                            __shfl(b, j, 64), __shfl(c, j, 64), __shfl(thr, j, 64));
+      if ((int)(threadIdx.x & 63) == j) count = tot;
+    }
+    if (live) {
       if (!pass) keys[inst] = 0xffffffffu;
       vals[inst] = (uint32_t)inst;
       cnt[inst] = count;
-      nvis += pass ? 1u : 0u;
     }
+    nvis += pass ? 1u : 0u;
   }
   // one atomic per wave
   uint32_t tot = nvis;

@@ -111,6 +111,46 @@ __device__ __forceinline__ uint32_t count_tiles(const uint32_t bb[4], int tw, fl
   return hit;
 }
 
+// Row and column of linear index k in a bbox of width bw (row-major), with
+// inv_bw = 1/bw: a float estimate of k / bw corrected to the exact quotient.
+__device__ __forceinline__ void bbox_rc(uint32_t k, uint32_t bw, float inv_bw, uint32_t* row, uint32_t* col) {
+  int32_t q = (int32_t)((float)k * inv_bw);
+  int32_t r = (int32_t)k - q * (int32_t)bw;
+  if (r < 0) {
+    --q;
+    r += (int32_t)bw;
+  } else if (r >= (int32_t)bw) {
+    ++q;
+    r -= (int32_t)bw;
+  }
+  *row = (uint32_t)q;
+  *col = (uint32_t)r;
+}
+
+// Tiles of a bbox passing the tile test, counted by the whole wave (all 64
+// lanes active, arguments wave-uniform): 64 bbox tiles per step, in the
+// reference's row-major order.  Used for bboxes of more than 64 tiles, whose
+// per-lane loop would stall the wave.
+__device__ __forceinline__ uint32_t wave_count_tiles(const uint32_t bb[4], int tw, float mx, float my, float a,
+                                                     float b, float c, float thr) {
+  const uint32_t bw = bb[2] - bb[0];
+  const uint32_t nb = (bb[3] - bb[1]) * bw;
+  const float inv = 1.0f / (float)bw;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t total = 0;
+  for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
+    const uint32_t k = k0 + lane;
+    bool hit = false;
+    if (k < nb) {
+      uint32_t row, col;
+      bbox_rc(k, bw, inv, &row, &col);
+      hit = tile_contributes(bb[0] + col, bb[1] + row, tw, mx, my, a, b, c, thr);
+    }
+    total += (uint32_t)__popcll(__ballot(hit));
+  }
+  return total;
+}
+
 __device__ __forceinline__ int find_view(const ViewDev* __restrict__ views, int nv, uint32_t tile) {
   int v = 0;
   for (int i = 1; i < nv; ++i)

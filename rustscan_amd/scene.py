@@ -36,23 +36,48 @@ class Rng:
         self.seed = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
         self.counter = 0
 
-    def u64(self, n: int) -> np.ndarray:
+    _CHUNK = 1 << 20
+
+    def _mix(self, start: int, n: int) -> np.ndarray:
         with np.errstate(over="ignore"):
-            idx = np.arange(self.counter + 1, self.counter + n + 1, dtype=np.uint64)
-            z = self.seed + idx * _GOLDEN
-            z = (z ^ (z >> np.uint64(30))) * _M1
-            z = (z ^ (z >> np.uint64(27))) * _M2
-            z = z ^ (z >> np.uint64(31))
-        self.counter += n
+            z = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+            z *= _GOLDEN
+            z += self.seed
+            z ^= z >> np.uint64(30)
+            z *= _M1
+            z ^= z >> np.uint64(27)
+            z *= _M2
+            z ^= z >> np.uint64(31)
         return z
 
+    def u64(self, n: int) -> np.ndarray:
+        out = np.empty(n, np.uint64)
+        for a in range(0, n, self._CHUNK):
+            b = min(n, a + self._CHUNK)
+            out[a:b] = self._mix(self.counter + a, b - a)
+        self.counter += n
+        return out
+
     def uniform(self, n: int) -> np.ndarray:
-        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        out = np.empty(n, np.float64)
+        for a in range(0, n, self._CHUNK):
+            b = min(n, a + self._CHUNK)
+            z = self._mix(self.counter + a, b - a)
+            z >>= np.uint64(11)
+            out[a:b] = z
+        out *= 1.0 / 9007199254740992.0
+        self.counter += n
+        return out
 
     def normal(self, n: int) -> np.ndarray:
-        u = self.uniform(2 * n)
-        u1, u2 = u[0::2], u[1::2]
-        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+        out = np.empty(n, np.float64)
+        half = self._CHUNK // 2
+        for a in range(0, n, half):
+            b = min(n, a + half)
+            u = self.uniform(2 * (b - a))
+            u1, u2 = u[0::2], u[1::2]
+            out[a:b] = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+        return out
 
 
 def sh_coeffs(degree: int) -> int:
