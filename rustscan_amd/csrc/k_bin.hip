@@ -411,16 +411,44 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
 }
 
 // get_tile_offsets.wgsl:12-30 over sorted keys: ranges[2t] = start, ranges[2t+1] = end.
+// Each thread scans 16 consecutive keys (vector loads) and writes the range
+// boundaries that fall inside them.
+constexpr uint32_t kRangeKeys = 16;
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ counters, uint32_t cap,
                                                      uint32_t num_tiles, uint32_t* __restrict__ ranges) {
   const uint32_t m = min(counters[kCntIsect], cap);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const uint32_t t = keys[i];
-  if (t >= num_tiles) return;
-  if (i == 0 || keys[i - 1] != t) ranges[2 * t] = i;
-  if (i + 1 == m || keys[i + 1] != t) ranges[2 * t + 1] = i + 1;
+  const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kRangeKeys;
+  if (i0 >= m) return;
+  uint32_t k[kRangeKeys];
+  if (i0 + kRangeKeys <= m) {
+    const uint4* src = reinterpret_cast<const uint4*>(keys + i0);
+#pragma unroll
+    for (uint32_t q = 0; q < kRangeKeys / 4; ++q) {
+      const uint4 x = src[q];
+      k[4 * q] = x.x;
+      k[4 * q + 1] = x.y;
+      k[4 * q + 2] = x.z;
+      k[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kRangeKeys; ++j) k[j] = i0 + j < m ? keys[i0 + j] : 0xffffffffu;
+  }
+  uint32_t prev = i0 == 0 ? 0xffffffffu : keys[i0 - 1];
+  const uint32_t after = i0 + kRangeKeys < m ? keys[i0 + kRangeKeys] : 0xffffffffu;
+#pragma unroll
+  for (uint32_t j = 0; j < kRangeKeys; ++j) {
+    const uint32_t i = i0 + j;
+    if (i >= m) break;
+    const uint32_t t = k[j];
+    const uint32_t next = i + 1 < m ? (j + 1 < kRangeKeys ? k[j + 1] : after) : 0xffffffffu;
+    if (t < num_tiles) {
+      if (t != prev) ranges[2 * t] = i;
+      if (t != next) ranges[2 * t + 1] = i + 1;
+    }
+    prev = t;
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -520,7 +548,8 @@ void launch_emit(hipStream_t s, const ProjSplat* proj, const uint32_t* order, co
 void launch_tile_ranges(hipStream_t s, const uint32_t* keys, const uint32_t* counters, uint32_t cap,
                         uint32_t num_tiles, uint32_t* ranges) {
   if (cap == 0) return;
-  k_tile_ranges<<<(cap + 255) / 256, 256, 0, s>>>(keys, counters, cap, num_tiles, ranges);
+  const uint64_t threads = ((uint64_t)cap + kRangeKeys - 1) / kRangeKeys;
+  k_tile_ranges<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(keys, counters, cap, num_tiles, ranges);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {

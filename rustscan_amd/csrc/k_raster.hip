@@ -22,10 +22,12 @@
 //
 // Backward: per splat the 64 lanes of a wave reduce their 12 contributions
 // with a transposed butterfly (permlane32 swap, permlane16 swap, then four
-// DPP row shifts), which leaves the 12 totals in lanes 15/31/47/63; those add
-// them into an LDS accumulator for the batch.  After the batch the totals go
-// to the per-instance accumulator with global atomics, one set per tile and
-// splat, issued so that the lanes of one instruction hit one 64-byte line.
+// DPP row shifts), which leaves the 12 totals in lanes 15/31/47/63 (the
+// value-to-lane map of the butterfly is measured once per kernel by reducing
+// tags); those lanes add them into an LDS accumulator for the batch.  After
+// the batch the totals go to the per-instance accumulator with global
+// atomics, one set per tile and splat, issued so that the lanes of one
+// instruction hit one 64-byte line.
 #include "rgs_device.h"
 
 namespace rgs {
@@ -76,32 +78,50 @@ __device__ __forceinline__ float row_reduce_l15(float v) {
   return v;
 }
 
-// a <- (a.lanes[0:32] ++ b.lanes[0:32]) + (a.lanes[32:64] ++ b.lanes[32:64]):
-// lanes 0-31 hold half-sums of a, lanes 32-63 half-sums of b.
+// Exchanges one 32-lane half of a with the other half of b and adds: one half
+// of the wave then holds pairwise sums of a's lanes, the other half b's.
+// The empty asm makes the two swap results opaque: without it this compiler
+// folds r[0] + r[1] into 2 * r[0].
 __device__ __forceinline__ float fold32(float a, float b) {
   const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
                                                   false, false);
-  return __builtin_bit_cast(float, r[0]) + __builtin_bit_cast(float, r[1]);
+  uint32_t r0 = r[0], r1 = r[1];
+  asm volatile("" : "+v"(r0), "+v"(r1));
+  return __builtin_bit_cast(float, r0) + __builtin_bit_cast(float, r1);
 }
 
-// Same with 16-lane rows: rows 0 and 2 keep a's row pairs (0+1, 2+3), rows 1
-// and 3 b's.
+// Same with 16-lane rows: two rows then hold row-pair sums of a, two of b.
 __device__ __forceinline__ float fold16(float a, float b) {
   const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
                                                   false, false);
-  return __builtin_bit_cast(float, r[0]) + __builtin_bit_cast(float, r[1]);
+  uint32_t r0 = r[0], r1 = r[1];
+  asm volatile("" : "+v"(r0), "+v"(r1));
+  return __builtin_bit_cast(float, r0) + __builtin_bit_cast(float, r1);
 }
 
-// Wave-wide sums of val[0..11].  On return, lane 16*r + 15 holds in out[a]
-// the total of val[a + 3*r] (a = 0..2, r = 0..3).  Every lane of the wave
-// must be active.
+// Wave-wide sums of val[0..11]: after the two folds each of the three
+// registers holds partial sums of four different values, one per 16-lane
+// row, and the row reductions leave the twelve totals in lanes 15/31/47/63
+// (three per lane).  Which value lands in which (lane, register) is fixed by
+// the swap pattern; wave_reduce12_slots() measures it once per kernel by
+// reducing the tags 1..12.  Every lane of the wave must be active.
 __device__ __forceinline__ void wave_reduce12(const float val[12], float out[3]) {
   float h[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) h[i] = fold32(val[i], val[i + 6]);  // rows 0,1: val[i]; rows 2,3: val[i+6]
+  for (int i = 0; i < 6; ++i) h[i] = fold32(val[i], val[i + 6]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) out[a] = row_reduce_l15(fold16(h[a], h[a + 3]));
-  // after fold16, row 0: val[a], row 1: val[a+3], row 2: val[a+6], row 3: val[a+9]
+}
+
+// Index of the value whose total wave_reduce12 leaves in out[a] of this lane
+// (meaningful in lanes 15/31/47/63): tag k+1 on every lane sums to 64(k+1).
+__device__ __forceinline__ void wave_reduce12_slots(int slot[3]) {
+  float tag[12], t[3];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) tag[k] = (float)(k + 1);
+  wave_reduce12(tag, t);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) slot[a] = (int)(t[a] * (1.0f / 64.0f) + 0.5f) - 1;
 }
 
 template <int TW, int BATCH, bool MULTI, bool COUNT>
@@ -227,6 +247,11 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
   const int lane = tid & 63;
   uint32_t ox, oy;
   pixel_in_tile<TW>(tid, &ox, &oy);
+  int slot[3];
+  wave_reduce12_slots(slot);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (!REFINE && slot[a] == 9) slot[a] = 12;  // coverage count rides in the refine slot
   uint32_t iter = 0, item;
   while (next_item(A, &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item;
@@ -351,14 +376,9 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         float tot[3];
         wave_reduce12(val, tot);
         if ((lane & 15) == 15) {
-          const int r = lane >> 4;
           float* dst = sacc + j * SA;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const int k = a + 3 * r;
-            const int slot = (!REFINE && k == 9) ? 12 : k;
-            atomicAdd(dst + slot, tot[a]);
-          }
+          for (int a = 0; a < 3; ++a) atomicAdd(dst + slot[a], tot[a]);
           if (REFINE && lane == 63) atomicAdd(dst + 12, (float)__popcll(m));
         }
       }
