@@ -39,6 +39,32 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+# kernel of each timed stage in the batched step (the dominant one is looked up
+# in the HBM-traffic profile)
+STAGE_KERNELS = {
+    "project_fwd": "k_project_fwd<3>", "depth_sort": "k_radix_scatter", "count_scan": "k_scan_down",
+    "emit": "k_emit", "tile_sort": "k_radix_scatter", "tile_ranges": "k_tile_ranges",
+    "raster_fwd": "k_raster_fwd<{tw}, {bd}, true, false>", "raster_bwd": "k_raster_bwd<{tw}, {bd}, false, true, false>",
+    "project_bwd": "k_project_bwd<3>", "adam": "k_adam",
+}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_8view_1M.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy), or None."""
+    import re
+
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for name, val in re.findall(r"traffic\[(.+?)\] = (\d+)", text):
+        if name == kernel:
+            return int(val)
+    return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -181,16 +207,20 @@ def main() -> int:
         # algorithmic bytes (HBM-bound stages) / flops (compositing) per launch
         sort_passes_depth = 4
         sort_passes_tile = (max(1, int(np.ceil(np.log2(max(2, sum(c[2] for c in counts)))))) + 7) // 8
+        T = sum(c[2] for c in counts)
         work = {
-            "project_fwd": ("hbm", (44 + 12 * k) * n + (48 + 12) * n * nv),
+            # parameters read once per Gaussian; ProjSplat + key + value + count written per instance
+            "project_fwd": ("hbm", (11 + 3 * k) * 4 * n + (64 + 12) * n * nv),
             "depth_sort": ("hbm", 20 * n * nv * sort_passes_depth),
             "count_scan": ("hbm", 12 * n * nv),
-            "emit": ("hbm", 52 * V + 8 * M),
+            # ProjSplat + order + offset read per visible instance; (key, value) written per intersection
+            "emit": ("hbm", 72 * V + 8 * M),
             "tile_sort": ("hbm", 20 * M * sort_passes_tile),
-            "tile_ranges": ("hbm", 12 * M),
+            "tile_ranges": ("hbm", 4 * M + 8 * T),
             "raster_fwd": ("mfma", 13 * frags.get("fwd_evaluated", 0) + 9 * frags.get("fwd_blended", 0)),
             "raster_bwd": ("mfma", 13 * frags.get("bwd_evaluated", 0) + 82 * frags.get("bwd_contributing", 0)),
-            "project_bwd": ("hbm", (44 + 64) * n * nv + (11 + 3 * k + 7) * 4 * n),
+            # accumulator + key per instance; parameters read, gradients and statistics written per Gaussian
+            "project_bwd": ("hbm", 68 * n * nv + ((11 + 3 * k) * 8 + 28) * n),
             "adam": ("hbm", 28 * (11 + 3 * k) * n),
         }
         for name, ms in per_launch.items():
@@ -205,6 +235,8 @@ def main() -> int:
                                            unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4))
         dom = max(per_launch, key=lambda s: per_launch[s])
         d = stage_summary[dom]
+        kname = STAGE_KERNELS[dom].format(tw=tw, bd=bd)
+        traffic = measured_traffic(kname)
         roofline = {
             "kernel": dom,
             "bound": "hbm" if d["bound"] == "hbm" else "mfma",
@@ -212,7 +244,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS if d["bound"] == "hbm" else FP32_PEAK_TFLOPS,
             "unit": d["unit"],
             "frac": d["frac"],
-            "traffic": None,
+            "traffic": traffic,
+            "kernel_name": kname,
             "ms_per_launch": d["ms"],
             "note": "fp32 vector ALU (no MFMA) for compositing; peak is the published FP32 figure"
             if d["bound"] != "hbm" else "algorithmic bytes per launch / HIP-event launch time",
