@@ -42,8 +42,8 @@ FP32_PEAK_TFLOPS = 157.3
 # kernel of each timed stage in the batched step (the dominant one is looked up
 # in the HBM-traffic profile)
 STAGE_KERNELS = {
-    "project_fwd": "k_project_fwd<3>", "depth_sort": "k_radix_scatter", "count_scan": "k_scan_down",
-    "emit": "k_emit", "tile_sort": "k_radix_scatter", "tile_ranges": "k_tile_ranges",
+    "project_fwd": "k_project_fwd<3>", "depth_sort": "k_radix_onesweep", "count_scan": "k_scan_down",
+    "emit": "k_emit", "tile_sort": "k_radix_onesweep", "tile_ranges": "k_tile_ranges",
     "raster_fwd": "k_raster_fwd<{tw}, {bd}, true, false>", "raster_bwd": "k_raster_bwd<{tw}, {bd}, false, true, false>",
     "project_bwd": "k_project_bwd<3>", "adam": "k_adam",
 }
@@ -76,6 +76,46 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a))
 
 
+def run_config5(args, ctx, opt) -> dict:
+    """Large batched training step: 4,194,304 Gaussians (1% with scale x20),
+    16 views at 1920x1080, uniform random targets; steps/s, intersections and
+    device memory in use."""
+    import torch
+
+    from rustscan_amd import scene
+    from rustscan_amd.rasterizer import DeviceSplats, Trainer
+
+    t0 = time.perf_counter()
+    sc = scene.config5_scene()
+    cams = scene.config5_cameras()
+    gen = torch.Generator(device="cuda").manual_seed(10)
+    targets = [torch.rand((c.height, c.width, 3), device="cuda", generator=gen) for c in cams]
+    splats = DeviceSplats(ctx, sc)
+    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams))
+    log(f"config5 setup {time.perf_counter() - t0:.1f}s")
+    trainer.train_step(splats, cams, targets, batched=True, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.config5_steps):
+        trainer.train_step(splats, cams, targets, batched=True, graph=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.config5_steps
+    counts = ctx.counts(len(cams))
+    free, total = torch.cuda.mem_get_info()
+    out = {
+        "workload": "4,194,304 Gaussians SH3 (1% log-scale + ln 20), 16 views 1920x1080, batched, HIP graph",
+        "steps_per_s": round(1.0 / dt, 4),
+        "ms_per_step": round(1e3 * dt, 2),
+        "intersections": int(sum(c[1] for c in counts)),
+        "visible_instances": int(sum(c[0] for c in counts)),
+        "device_memory_in_use_gb": round((total - free) / 1e9, 2),
+        "skipped_steps": ctx.skipped_steps(),
+    }
+    log(f"config5: {out}")
+    splats.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +126,10 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-instrument", action="store_true")
+    ap.add_argument("--config5", action="store_true",
+                    help="also measure the large configuration (4,194,304 Gaussians with a heavy scale tail, "
+                         "16 views at 1920x1080 per batched step)")
+    ap.add_argument("--config5-steps", type=int, default=3)
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--batch-depth", type=int, default=256)
     args = ap.parse_args()
@@ -211,14 +255,16 @@ def main() -> int:
         work = {
             # parameters read once per Gaussian; ProjSplat + key + value + count written per instance
             "project_fwd": ("hbm", (11 + 3 * k) * 4 * n + (64 + 12) * n * nv),
-            "depth_sort": ("hbm", 20 * n * nv * sort_passes_depth),
+            "depth_sort": ("hbm", 16 * n * nv * sort_passes_depth + 4 * n * nv),
             "count_scan": ("hbm", 12 * n * nv),
             # ProjSplat + order + offset read per visible instance; (key, value) written per intersection
             "emit": ("hbm", 72 * V + 8 * M),
-            "tile_sort": ("hbm", 20 * M * sort_passes_tile),
+            "tile_sort": ("hbm", 16 * M * sort_passes_tile + 4 * M),
             "tile_ranges": ("hbm", 4 * M + 8 * T),
-            "raster_fwd": ("mfma", 13 * frags.get("fwd_evaluated", 0) + 9 * frags.get("fwd_blended", 0)),
-            "raster_bwd": ("mfma", 13 * frags.get("bwd_evaluated", 0) + 82 * frags.get("bwd_contributing", 0)),
+            # 28 FP32 operations per evaluated fragment (sigma 11, exp 1, alpha 2, tests 2, blend 9,
+            # transmittance 3); about 95 per contributing fragment in the backward
+            "raster_fwd": ("mfma", 28 * frags.get("fwd_evaluated", 0)),
+            "raster_bwd": ("mfma", 95 * frags.get("bwd_contributing", 0)),
             # accumulator + key per instance; parameters read, gradients and statistics written per Gaussian
             "project_bwd": ("hbm", 68 * n * nv + ((11 + 3 * k) * 8 + 28) * n),
             "adam": ("hbm", 28 * (11 + 3 * k) * n),
@@ -233,6 +279,12 @@ def main() -> int:
                 ach = amount / (ms * 1e-3) / 1e12
                 stage_summary[name] = dict(ms=round(ms, 4), bound="fp32-valu", achieved=round(ach, 2),
                                            unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 4))
+        if "raster_fwd" in per_launch and frags:
+            stage_summary["raster_fwd"]["blended_fragments_per_s"] = round(
+                frags.get("fwd_blended", 0) / (per_launch["raster_fwd"] * 1e-3), 1)
+        if "raster_bwd" in per_launch and frags:
+            stage_summary["raster_bwd"]["contributing_fragments_per_s"] = round(
+                frags.get("bwd_contributing", 0) / (per_launch["raster_bwd"] * 1e-3), 1)
         dom = max(per_launch, key=lambda s: per_launch[s])
         d = stage_summary[dom]
         kname = STAGE_KERNELS[dom].format(tw=tw, bd=bd)
@@ -250,6 +302,20 @@ def main() -> int:
             "note": "fp32 vector ALU (no MFMA) for compositing; peak is the published FP32 figure"
             if d["bound"] != "hbm" else "algorithmic bytes per launch / HIP-event launch time",
         }
+
+    # ---- the reference's CPU renderer (render/tiled_renderer.rs restated, single-threaded)
+    cpu_render = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        cam = cams[2]  # 1280x720
+        t0 = time.perf_counter()
+        O.tiled_renderer(state, cam, cov_blur=0.3)
+        t_r = time.perf_counter() - t0
+        cpu_render = {"value": round(cam.width * cam.height / t_r / 1e6, 4), "unit": "Mpix/s", "cores": 1,
+                      "kind": "port", "sample": f"one {cam.width}x{cam.height} forward render of the 1,048,576-"
+                                                "Gaussian scene by the CPU tiled renderer"}
+        log(f"cpu tiled renderer: {t_r:.1f}s for one {cam.width}x{cam.height} view")
 
     # ---- CPU baseline: the oracle for one full step, and gradient agreement with it
     cpu = None
@@ -322,6 +388,8 @@ def main() -> int:
         "fragments_per_step": frags or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_tiled_renderer": cpu_render,
+        "config5": run_config5(args, ctx, opt) if args.config5 else None,
         "device": dev_name,
     }
     print(json.dumps(out), flush=True)

@@ -72,7 +72,9 @@ enum { RGS_GRAPH_OFF = 0, RGS_GRAPH_ON = 1 };
  * mean over H*W*3), backward summed over views, one Adam update. */
 typedef struct rgs_step_cfg {
   int mode;          /* RGS_MODE_PER_VIEW: views one after another, grads
-                        accumulated; RGS_MODE_BATCHED: all views per launch */
+                        accumulated; RGS_MODE_BATCHED: all views per launch
+                        (split into consecutive groups of views when one
+                        launch would exceed 1.6e9 intersections) */
   int graph;         /* RGS_GRAPH_ON: capture the step as a HIP graph on first
                         use and replay it while shapes and capacities hold */
   float cov_blur;    /* ProjectUniforms::cov_blur (clamped at 0) */
@@ -103,6 +105,10 @@ int rgs_sync(rgs_ctx* ctx);              /* blocks until the stream is idle */
 void* rgs_get_stream(rgs_ctx* ctx);      /* hipStream_t of the context */
 int rgs_set_raster_config(rgs_ctx* ctx, const rgs_raster_cfg* cfg);
 int rgs_get_raster_config(rgs_ctx* ctx, rgs_raster_cfg* cfg);
+/* Most intersections one batched launch may hold (default 1.6e9, at most
+ * 3e9); a batched step whose views exceed it is run as consecutive groups of
+ * views.  No reference counterpart: RustGS renders one view per launch. */
+int rgs_set_launch_limit(rgs_ctx* ctx, int64_t max_intersections);
 
 /* ---- splat store (replaces DeviceSplats, engine/splats.rs:13-63) ---- */
 int rgs_splats_create(rgs_ctx* ctx, int64_t n, int sh_degree, rgs_splats** out);

@@ -124,6 +124,7 @@ _SIGNATURES = {
     "rgs_get_fragment_counts": (c_int, [c_void_p, POINTER(c_int64), c_int]),
     "rgs_set_timing": (c_int, [c_void_p, c_int]),
     "rgs_get_timing": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_int64), c_int]),
+    "rgs_set_launch_limit": (c_int, [c_void_p, c_int64]),
     "rgs_version": (ctypes.c_char_p, []),
 }
 

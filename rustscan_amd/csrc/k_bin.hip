@@ -110,7 +110,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_spine(uint32_t* __restri
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev, uint32_t n_host, uint32_t cap, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, const uint32_t* __restrict__ gsrc,
+    uint32_t* __restrict__ gout) {
   __shared__ uint32_t base[256];     // next global slot of each digit for this block
   __shared__ uint32_t gbase[256];    // global slot of local position 0 of each digit's run in the tile
   __shared__ uint32_t wh[4][256];    // per-wave digit counts, then per-wave local bases
@@ -188,8 +189,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     for (uint32_t i = threadIdx.x; i < tn; i += kSortThreads) {
       const uint32_t k = sk[i];
       const uint32_t dst = gbase[(k >> shift) & 255u] + i;
+      const uint32_t v = sv[i];
       kout[dst] = k;
-      vout[dst] = sv[i];
+      vout[dst] = v;
+      if (gout) gout[dst] = gsrc[v];
     }
     __syncthreads();
   }
@@ -495,15 +498,17 @@ static uint32_t grid_for(uint64_t cap) {
 // The count is read from n_dev when non-null (clamped to cap).
 // Pass 0 reads (kin, vin) and writes (kb, vb); later passes ping-pong
 // between (kb, vb) and (kc, vc).  (kc, vc) may alias (kin, vin) when the input
-// may be clobbered.  vin == nullptr means values are the input indices.
+// may be clobbered.  vin == nullptr means values are the input indices.  When
+// gout is non-null the last pass also writes gout[i] = gsrc[value[i]].
 // Returns 1 if the result is in (kb, vb), 2 if in (kc, vc), 0 if key_bits <= 0
 // (nothing done; the input order is already sorted).
-// scratch: 256 * G + 256 uint32 with G = radix_grid(cap).
+// scratch: radix_scratch_words(cap) uint32.
 uint32_t radix_grid(uint64_t cap) { return grid_for<kRadixTile>(cap); }
+uint64_t radix_scratch_words(uint64_t cap) { return 256ull * radix_grid(cap) + 512; }
 
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
-                     uint32_t* scratch) {
+                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout) {
   if (key_bits <= 0) return 0;
   const uint32_t G = grid_for<kRadixTile>(cap);
   uint32_t* hist = scratch;
@@ -515,9 +520,11 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
   for (int p = 0; p < passes; ++p) {
     uint32_t* dk = (p % 2 == 0) ? kb : kc;
     uint32_t* dv = (p % 2 == 0) ? vb : vc;
+    const bool last = p + 1 == passes;
     k_radix_hist<<<G, kSortThreads, 0, s>>>(sk, n_dev, n_host, cap, 8 * p, hist);
     k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot);
-    k_radix_scatter<<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot);
+    k_radix_scatter<<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot,
+                                               last ? gsrc : nullptr, last ? gout : nullptr);
     sk = dk;
     sv = dv;
     where = (p % 2 == 0) ? 1 : 2;
@@ -527,7 +534,7 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
 
 // Exclusive (or inclusive) scan of in[gather[i]] (gather may be null) for
 // i < n; the grand total is written to *total_dev when non-null.
-// scratch: grid_for(cap) <= 4 * radix_grid(cap) uint32.
+// scratch: grid_for(cap) uint32 (<= 2048).
 void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const uint32_t* n_dev, uint32_t n_host,
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev,
               uint32_t* scratch) {

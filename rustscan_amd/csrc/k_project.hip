@@ -383,11 +383,14 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     const size_t inst = (size_t)v * n + gid;
     if (keys_by_inst[inst] == 0xffffffffu) continue;  // culled in this view
     float4* ap = reinterpret_cast<float4*>(acc + inst * kAcc);
-    const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3];
+    const float4 a3 = ap[3];
+    // coverage 0: the rasterizer never flushed into this line, which is
+    // still all zero, and every output term is zero
+    if (a3.x == 0.0f) continue;
+    const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2];
     // consume and clear so the accumulator starts at zero next step
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     ap[0] = z4; ap[1] = z4; ap[2] = z4; ap[3] = z4;
-    if (a3.x == 0.0f) continue;  // no contributing fragment: every output term is zero
     any = true;
     const ViewDev& V = views[v];
     const float vxy[2] = {a0.x, a0.y};

@@ -32,8 +32,6 @@
 
 namespace rgs {
 
-constexpr float kSkipMargin = 1e-3f;
-
 __device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t* s_item, uint32_t& iter, uint32_t* item) {
   if (A.queue) {
     if (threadIdx.x == 0) *s_item = atomicAdd(A.queue, 1u);

@@ -25,9 +25,10 @@ void launch_project_bwd(int deg, hipStream_t s, const float* P, uint32_t npad, u
                         int v0, int nv, float blur, float* acc, const uint32_t* keys_by_inst, float* grads,
                         float* stats, int accumulate, const uint32_t* counters);
 uint32_t radix_grid(uint64_t cap);
+uint64_t radix_scratch_words(uint64_t cap);
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
-                     uint32_t* scratch);
+                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout);
 void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const uint32_t* n_dev, uint32_t n_host,
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev,
               uint32_t* scratch);
@@ -86,7 +87,7 @@ struct Workspace {
   size_t cap_inst = 0;   // n * view_slots
   size_t cap_isect = 0;  // intersection capacity
   uint32_t tiles_cap = 0;
-  DevBuf views, proj, keys, vals, cnt, sk1, sv1, sk2, sv2, offs, ik0, iv0, ik1, iv1, ranges, acc, acc_dbg, counters,
+  DevBuf views, proj, keys, vals, cnt, scnt, sk1, sv1, sk2, sv2, offs, ik0, iv0, ik1, iv1, ranges, acc, acc_dbg, counters,
       fscal, scratch;
   std::vector<DevBuf> own_img;  // images kept internal when the caller passes none
 };
@@ -133,6 +134,10 @@ struct rgs_ctx {
   StepPlanKey gkey{};
   const rgs_splats* gsplats = nullptr;
   std::map<StepPlanKey, bool> sized;
+  // batched mode: view groups [starts[i], starts[i+1]) launched together, so
+  // that no launch holds more than launch_limit intersections
+  std::map<StepPlanKey, std::vector<int>> groups;
+  uint64_t launch_limit = 1600000000ull;
   // host mirrors
   uint32_t* h_counters = nullptr;
   float* h_fscal = nullptr;
@@ -281,6 +286,7 @@ int ensure_workspace(rgs_ctx* c, const rgs_splats* s, int view_slots, uint32_t t
     rc |= ensure(c, w.keys, ci * 4);
     rc |= ensure(c, w.vals, ci * 4);
     rc |= ensure(c, w.cnt, ci * 4);
+    rc |= ensure(c, w.scnt, ci * 4);
     rc |= ensure(c, w.sk1, ci * 4);
     rc |= ensure(c, w.sv1, ci * 4);
     rc |= ensure(c, w.sk2, ci * 4);
@@ -312,7 +318,7 @@ int ensure_workspace(rgs_ctx* c, const rgs_splats* s, int view_slots, uint32_t t
     w.tiles_cap = tiles;
     grew = true;
   }
-  const size_t scratch = 256ull * radix_grid(std::max(w.cap_inst, w.cap_isect)) + 512;
+  const size_t scratch = std::max<size_t>(radix_scratch_words(std::max(w.cap_inst, w.cap_isect)), 4096);
   if (ensure(c, w.scratch, scratch * 4)) return RGS_ERR_HIP;
   if (!w.views.p) {
     if (ensure(c, w.views, sizeof(ViewDev) * kMaxViews)) return RGS_ERR_HIP;
@@ -378,15 +384,16 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   int wd;
   {
     StageTimer t(c, RGS_STAGE_DEPTH_SORT);
+    // the last pass also gathers the tile counts into depth order (scnt)
     wd = radix_sort_pairs(st, w.keys.as<uint32_t>() + ibase, w.vals.as<uint32_t>() + ibase, w.sk1.as<uint32_t>(),
                           w.sv1.as<uint32_t>(), w.sk2.as<uint32_t>(), w.sv2.as<uint32_t>(), nullptr, nitems, nitems,
-                          32, w.scratch.as<uint32_t>());
+                          32, w.scratch.as<uint32_t>(), w.cnt.as<uint32_t>(), w.scnt.as<uint32_t>());
   }
   uint32_t* order = wd == 1 ? w.sv1.as<uint32_t>() : w.sv2.as<uint32_t>();
   // exclusive scan of per-instance tile counts in depth order (prefix_sum.rs:53-136)
   {
     StageTimer t(c, RGS_STAGE_COUNT_SCAN);
-    scan_u32(st, w.cnt.as<uint32_t>(), order, counters + kCntVisible, 0, nitems, w.offs.as<uint32_t>(), 0,
+    scan_u32(st, w.scnt.as<uint32_t>(), nullptr, counters + kCntVisible, 0, nitems, w.offs.as<uint32_t>(), 0,
              counters + kCntIsect, counters + kCntIsectMax, w.scratch.as<uint32_t>());
   }
   // emission (map_gaussian_to_intersects.wgsl) and stable tile sort (tile_mapping.rs:206-215)
@@ -401,7 +408,7 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
     StageTimer t(c, RGS_STAGE_TILE_SORT);
     wt = radix_sort_pairs(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(), w.iv1.as<uint32_t>(),
                           w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect, 0, (uint32_t)w.cap_isect,
-                          bits, w.scratch.as<uint32_t>());
+                          bits, w.scratch.as<uint32_t>(), nullptr, nullptr);
   }
   uint32_t* skeys = wt == 1 ? w.ik1.as<uint32_t>() : w.ik0.as<uint32_t>();
   uint32_t* svals = wt == 1 ? w.iv1.as<uint32_t>() : w.iv0.as<uint32_t>();
@@ -496,15 +503,21 @@ int enqueue_adam(rgs_ctx* c, rgs_splats* s, const rgs_adam_cfg* cfg, int nviews,
   return RGS_OK;
 }
 
-int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cfg* cfg) {
+int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cfg* cfg,
+                 const std::vector<int>* groups) {
   const int nv = (int)vs.v.size();
   const float blur = cfg->cov_blur > 0.0f ? cfg->cov_blur : 0.0f;
   launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
   int rc = RGS_OK;
   if (cfg->mode == RGS_MODE_BATCHED) {
-    if ((rc = enqueue_forward(c, s, vs, 0, nv, blur, true))) return rc;
-    if ((rc = enqueue_backward(c, s, vs, 0, nv, blur, false, false))) return rc;
+    const std::vector<int> all = {0, nv};
+    const std::vector<int>& g = (groups && groups->size() >= 2) ? *groups : all;
+    for (size_t i = 0; i + 1 < g.size(); ++i) {
+      const int v0 = g[i], cnt = g[i + 1] - g[i];
+      if ((rc = enqueue_forward(c, s, vs, v0, cnt, blur, true))) return rc;
+      if ((rc = enqueue_backward(c, s, vs, v0, cnt, blur, i > 0, false))) return rc;
+    }
   } else {
     for (int v = 0; v < nv; ++v) {
       if ((rc = enqueue_forward(c, s, vs, v, 1, blur, true))) return rc;
@@ -524,28 +537,70 @@ int read_counters(rgs_ctx* c) {
   return RGS_OK;
 }
 
-// Run forwards only (no parameter change) to learn the intersection count,
-// then grow the capacity with headroom.
-int size_for(rgs_ctx* c, rgs_splats* s, ViewSet& vs, int mode, float blur) {
+// Run forwards only (no parameter change) to learn the intersection counts,
+// then grow the capacity with headroom.  In batched mode the views are first
+// rendered one at a time to learn each view's count, and grouped so that no
+// launch exceeds c->launch_limit intersections (a batch's count is the sum
+// of its views' counts); *groups receives the group starts and the view count.
+
+int size_for(rgs_ctx* c, rgs_splats* s, ViewSet& vs, int mode, float blur, std::vector<int>* groups) {
+  const int nv = (int)vs.v.size();
+  int rc = 0;
+  uint64_t need = 0;
+  if (mode == RGS_MODE_BATCHED && nv > 1) {
+    std::vector<uint64_t> per(nv, 0);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+      if ((rc = upload_views(c, vs))) return rc;
+      uint64_t worst = 0;
+      bool over = false;
+      for (int v = 0; v < nv; ++v) {
+        launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
+        if ((rc = enqueue_forward(c, s, vs, v, 1, blur, false))) return rc;
+        if ((rc = read_counters(c))) return rc;
+        per[v] = c->h_counters[kCntIsect];
+        worst = std::max<uint64_t>(worst, per[v]);
+        over = over || c->h_counters[kCntOverflow] != 0;
+      }
+      if (!over) break;
+      if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, worst * 5 / 4 + 1024))) return rc;
+      if (attempt == 2) return fail(c, RGS_ERR_CAPACITY, "could not size the intersection buffers");
+    }
+    groups->assign(1, 0);
+    uint64_t run = 0;
+    for (int v = 0; v < nv; ++v) {
+      if (v > groups->back() && run + per[v] > c->launch_limit) {
+        need = std::max(need, run);
+        groups->push_back(v);
+        run = 0;
+      }
+      run += per[v];
+    }
+    need = std::max(need, run);
+    groups->push_back(nv);
+    if (need * 5 / 4 + 1024 > c->ws.cap_isect || need * 5 / 4 > c->ws.cap_isect) {
+      if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, need * 5 / 4 + 1024))) return rc;
+    }
+    return RGS_OK;
+  }
+  groups->assign({0, nv});
   for (int attempt = 0; attempt < 3; ++attempt) {
-    int rc = upload_views(c, vs);
-    if (rc) return rc;
-    launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), (int)vs.v.size());
+    if ((rc = upload_views(c, vs))) return rc;
+    launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
     if (mode == RGS_MODE_BATCHED) {
-      if ((rc = enqueue_forward(c, s, vs, 0, (int)vs.v.size(), blur, false))) return rc;
+      if ((rc = enqueue_forward(c, s, vs, 0, nv, blur, false))) return rc;
     } else {
-      for (int v = 0; v < (int)vs.v.size(); ++v)
+      for (int v = 0; v < nv; ++v)
         if ((rc = enqueue_forward(c, s, vs, v, 1, blur, false))) return rc;
     }
     if ((rc = read_counters(c))) return rc;
-    const uint64_t need = c->h_counters[kCntIsectMax];
+    need = c->h_counters[kCntIsectMax];
     if (need <= c->ws.cap_isect) {
       if (need * 5 / 4 > c->ws.cap_isect) {
-        if ((rc = ensure_workspace(c, s, (int)vs.v.size(), vs.tiles_total, need * 5 / 4 + 1024))) return rc;
+        if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, need * 5 / 4 + 1024))) return rc;
       }
       return RGS_OK;
     }
-    if ((rc = ensure_workspace(c, s, (int)vs.v.size(), vs.tiles_total, need * 5 / 4 + 1024))) return rc;
+    if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, need * 5 / 4 + 1024))) return rc;
   }
   return fail(c, RGS_ERR_CAPACITY, "could not size the intersection buffers");
 }
@@ -604,7 +659,7 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   invalidate_graph(c);
   Workspace& w = c->ws;
   release(c->frag);
-  DevBuf* bufs[] = {&w.views, &w.proj, &w.keys, &w.vals, &w.cnt, &w.sk1, &w.sv1, &w.sk2, &w.sv2, &w.offs, &w.ik0,
+  DevBuf* bufs[] = {&w.views, &w.proj, &w.keys, &w.vals, &w.cnt, &w.scnt, &w.sk1, &w.sv1, &w.sk2, &w.sv2, &w.offs, &w.ik0,
                     &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch};
   for (DevBuf* b : bufs) release(*b);
   for (auto& b : w.own_img) release(b);
@@ -641,6 +696,16 @@ int rgs_set_raster_config(rgs_ctx* c, const rgs_raster_cfg* cfg) {
 int rgs_get_raster_config(rgs_ctx* c, rgs_raster_cfg* cfg) {
   if (!c || !cfg) return RGS_ERR_INVALID;
   *cfg = c->raster;
+  return RGS_OK;
+}
+
+int rgs_set_launch_limit(rgs_ctx* c, int64_t max_intersections) {
+  if (!c || max_intersections < 1 || max_intersections > (int64_t)3000000000ll)
+    return c ? fail(c, RGS_ERR_INVALID, "launch limit out of range") : RGS_ERR_INVALID;
+  c->launch_limit = (uint64_t)max_intersections;
+  c->sized.clear();
+  c->groups.clear();
+  invalidate_graph(c);
   return RGS_OK;
 }
 
@@ -899,9 +964,10 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
   HIPCHK(c, hipMemcpyAsync(s->lr.p, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice, c->stream));
   const StepPlanKey key = plan_key(c, s, vs, cfg->mode);
   if (!c->sized[key]) {
-    if ((rc = size_for(c, s, vs, cfg->mode, blur))) return rc;
+    if ((rc = size_for(c, s, vs, cfg->mode, blur, &c->groups[key]))) return rc;
     c->sized[key] = true;
   }
+  const std::vector<int>* groups = &c->groups[key];
   for (int attempt = 0; attempt < 3; ++attempt) {
     if ((rc = upload_views(c, vs))) return rc;
     if (cfg->graph == RGS_GRAPH_ON) {
@@ -910,7 +976,7 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
         invalidate_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         c->capturing = true;
-        rc = enqueue_step(c, s, vs, cfg);
+        rc = enqueue_step(c, s, vs, cfg, groups);
         c->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -931,7 +997,7 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
       }
       HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     } else {
-      if ((rc = enqueue_step(c, s, vs, cfg))) return rc;
+      if ((rc = enqueue_step(c, s, vs, cfg, groups))) return rc;
     }
     if (!loss_out) break;
     // synchronous call: check the step and retry after growing on overflow
@@ -1030,32 +1096,31 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
 
 int rgs_get_counts(rgs_ctx* c, rgs_counts* out, int nv) {
   if (!c || !out) return RGS_ERR_INVALID;
-  if (!c->ws.counters.p) return fail(c, RGS_ERR_STATE, "no forward yet");
-  // Recount per view from the sorted intersection keys and depth order of the last batched forward.
-  int rc = read_counters(c);
-  if (rc) return rc;
+  if (!c->ws.counters.p || c->fwd_views.v.empty()) return fail(c, RGS_ERR_STATE, "no forward yet");
+  // Per view: visible instances from the depth keys (culled ones hold
+  // 0xffffffff), intersections from the tile ranges of the view's tiles.
+  // Both stay valid for every view of the last step, also when its views were
+  // launched in groups.
   const ViewSet& vs = c->fwd_views;
   if (nv > (int)vs.v.size()) return fail(c, RGS_ERR_INVALID, "n_views larger than the last forward");
-  const uint32_t V = c->h_counters[kCntVisible];
-  const uint32_t M = std::min<uint64_t>(c->h_counters[kCntIsect], c->ws.cap_isect);
-  std::vector<uint32_t> order(V), keys(M);
-  if (V) HIPCHK(c, hipMemcpy(order.data(), c->fwd_order, 4ull * V, hipMemcpyDeviceToHost));
-  if (M) HIPCHK(c, hipMemcpy(keys.data(), c->fwd_isect_keys, 4ull * M, hipMemcpyDeviceToHost));
-  for (int v = 0; v < nv; ++v) {
-    out[v].num_visible = 0;
-    out[v].num_intersections = 0;
-    out[v].num_tiles = (int64_t)vs.v[v].tiles_x * vs.v[v].tiles_y;
-  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   const uint32_t n = c->ws.n;
-  for (uint32_t i = 0; i < V; ++i) {
-    const int v = (int)(order[i] / n);
-    if (v >= c->fwd_v0 && v - c->fwd_v0 < nv) out[v - c->fwd_v0].num_visible++;
-  }
-  for (uint32_t i = 0; i < M; ++i) {
-    int v = 0;
-    for (int k = 0; k < (int)vs.v.size(); ++k)
-      if (keys[i] >= vs.v[k].tile_base) v = k;
-    if (v >= c->fwd_v0 && v - c->fwd_v0 < nv) out[v - c->fwd_v0].num_intersections++;
+  std::vector<uint32_t> keys(n);
+  for (int v = 0; v < nv; ++v) {
+    const ViewDev& V = vs.v[v];
+    const uint32_t T = V.tiles_x * V.tiles_y;
+    std::vector<uint32_t> rg(2ull * T);
+    if (T) HIPCHK(c, hipMemcpy(rg.data(), c->ws.ranges.as<uint32_t>() + 2ull * V.tile_base, 8ull * T,
+                               hipMemcpyDeviceToHost));
+    int64_t m = 0;
+    for (uint32_t t = 0; t < T; ++t) m += (int64_t)rg[2 * t + 1] - (int64_t)rg[2 * t];
+    if (n) HIPCHK(c, hipMemcpy(keys.data(), c->ws.keys.as<uint32_t>() + (size_t)v * n, 4ull * n,
+                               hipMemcpyDeviceToHost));
+    int64_t vis = 0;
+    for (uint32_t i = 0; i < n; ++i) vis += keys[i] != 0xffffffffu;
+    out[v].num_visible = vis;
+    out[v].num_intersections = m;
+    out[v].num_tiles = T;
   }
   return RGS_OK;
 }
@@ -1076,10 +1141,10 @@ int rgs_sort_pairs_u32(rgs_ctx* c, const uint32_t* keys, const uint32_t* values,
   int rc = 0;
   rc |= ensure(c, k2, (size_t)n * 4);
   rc |= ensure(c, v2, (size_t)n * 4);
-  rc |= ensure(c, scratch, (256ull * radix_grid((uint64_t)n) + 512) * 4);
+  rc |= ensure(c, scratch, radix_scratch_words((uint64_t)n) * 4);
   if (rc) return RGS_ERR_HIP;
   const int where = radix_sort_pairs(c->stream, keys, values, ok, ov, k2.as<uint32_t>(), v2.as<uint32_t>(), nullptr,
-                                     (uint32_t)n, (uint32_t)n, 32, scratch.as<uint32_t>());
+                                     (uint32_t)n, (uint32_t)n, 32, scratch.as<uint32_t>(), nullptr, nullptr);
   if (where == 2) {
     HIPCHK(c, hipMemcpyAsync(ok, k2.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(ov, v2.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
@@ -1098,7 +1163,7 @@ int rgs_inclusive_scan_u32(rgs_ctx* c, const uint32_t* in, uint32_t* out, int64_
   if (n >= (1ll << 31)) return fail(c, RGS_ERR_INVALID, "n too large");
   HIPCHK(c, hipSetDevice(c->device));
   DevBuf scratch;
-  if (ensure(c, scratch, (256ull * radix_grid((uint64_t)n) + 512) * 4)) return RGS_ERR_HIP;
+  if (ensure(c, scratch, 4096 * 4)) return RGS_ERR_HIP;
   scan_u32(c->stream, in, nullptr, nullptr, (uint32_t)n, (uint32_t)n, out, 1, nullptr, nullptr, scratch.as<uint32_t>());
   hipError_t e = hipStreamSynchronize(c->stream);
   release(scratch);

@@ -11,6 +11,11 @@ namespace rgs {
 // Constants are the reference's WGSL abstract-float literals rounded to f32.
 #define RGS_F(x) ((float)(x))
 
+// Margin of the compositing kernels' exp skip: a fragment with
+// sigma > log(255 * opacity) + kSkipMargin has alpha < 1/255 whatever exp
+// rounds to, so it is rejected without evaluating exp.
+constexpr float kSkipMargin = 1e-3f;
+
 __device__ __forceinline__ float fminf_(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float fmaxf_(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float clampf_(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }

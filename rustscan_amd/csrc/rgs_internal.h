@@ -32,8 +32,8 @@ struct ViewDev {
   const float* vout;
 };
 
-// Projected splat, 64 bytes (half a cache line), the unit every later stage
-// gathers by instance.
+// Projected splat instance, 64 bytes, the unit every later stage gathers by
+// instance.
 //   p0 = (x, y, a/2, b)        conic = [[a, b], [b, c]]
 //   p1 = (c/2, opacity, r, g)  colour before the max(., 0) clamp
 //   p2 = (blue, threshold, bbox_lo, bbox_hi)

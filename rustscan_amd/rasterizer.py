@@ -85,6 +85,10 @@ class Context:
         cfg = L.RgsRasterCfg(tile_width, batch_depth, 1 if persistent else 0)
         self.check(self.lib.rgs_set_raster_config(self.h, ctypes.byref(cfg)))
 
+    def set_launch_limit(self, max_intersections: int):
+        """Most intersections per batched launch; larger steps run as groups of views."""
+        self.check(self.lib.rgs_set_launch_limit(self.h, int(max_intersections)))
+
     def raster(self) -> tuple:
         cfg = L.RgsRasterCfg()
         self.check(self.lib.rgs_get_raster_config(self.h, ctypes.byref(cfg)))

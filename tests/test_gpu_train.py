@@ -102,6 +102,35 @@ def test_per_view_equals_batched(hip_ctx, setup):
     assert max(errs[:3]) <= 1e-5
 
 
+def test_view_groups_equal_one_launch(hip_ctx, setup):
+    """A launch limit below the step's intersection count splits the batched
+    step into consecutive groups of views; the gradients and losses equal the
+    single-launch step, eagerly and when captured as a graph."""
+    from rustscan_amd.rasterizer import DeviceSplats
+
+    sc, cams, _, tgt = setup
+    tr = make_trainer(hip_ctx, 1.0 / len(cams))
+    ds = DeviceSplats(hip_ctx, sc)
+    l_one = tr.train_step(ds, cams, tgt, batched=True, graph=False, read_loss=True)
+    g_one = ds.grads()
+    counts = hip_ctx.counts(len(cams))
+    m = [int(c[1]) for c in counts]
+    say(f"  intersections per view {m}")
+    try:
+        hip_ctx.set_launch_limit(max(m) + 1)  # every view in its own group
+        for graph in (False, True):
+            ds.upload(sc)
+            l_grp = tr.train_step(ds, cams, tgt, batched=True, graph=graph, read_loss=True)
+            g_grp = ds.grads()
+            errs = [rel_l2(a, b) for a, b in zip(g_grp, g_one)]
+            say(f"  grouped (graph={graph}) vs one launch " + " ".join(f"{e:.2e}" for e in errs))
+            assert max(errs[:3]) <= 1e-5
+            np.testing.assert_allclose(l_grp, l_one, rtol=1e-5)
+    finally:
+        hip_ctx.set_launch_limit(1600000000)
+    ds.close()
+
+
 def test_graph_replay_tracks_eager(hip_ctx, setup):
     """Each replay of the captured step computes the gradients an eager step
     computes from the same parameters; after the first step (same Adam state)
