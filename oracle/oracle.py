@@ -84,6 +84,7 @@ def lib():
         L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_exp_core.argtypes = [c_void_p, c_void_p, c_int64]
+        L.orc_math_exp_core_fast.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_sh_color.argtypes = [c_int, c_void_p, c_void_p, c_void_p]
         L.orc_sort_by_key.argtypes = [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]
         L.orc_inclusive_scan.argtypes = [c_int64, c_void_p, c_void_p]
@@ -284,6 +285,13 @@ def math_exp_core(x: np.ndarray) -> np.ndarray:
     x = _f32(x)
     y = np.zeros_like(x)
     lib().orc_math_exp_core(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def math_exp_core_fast(x: np.ndarray) -> np.ndarray:
+    x = _f32(x)
+    y = np.zeros_like(x)
+    lib().orc_math_exp_core_fast(x.ctypes.data, y.ctypes.data, x.size)
     return y
 
 

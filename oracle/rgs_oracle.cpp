@@ -1401,6 +1401,9 @@ void orc_math_exp(const float* x, float* y, int64_t n) {
 void orc_math_exp_core(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_exp_core(x[i]);
 }
+void orc_math_exp_core_fast(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = rgs_exp_core_fast(x[i]);
+}
 void orc_math_log(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_log(x[i]);
 }

@@ -122,6 +122,7 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions,
 void orc_math_exp(const float* x, float* y, int64_t n);
 void orc_math_log(const float* x, float* y, int64_t n);
 void orc_math_exp_core(const float* x, float* y, int64_t n);
+void orc_math_exp_core_fast(const float* x, float* y, int64_t n);
 /* SH colour (project_visible.wgsl:44-127): coeffs [K*3], viewdir[3] -> rgb[3] */
 void orc_sh_color(int degree, const float* coeffs, const float* viewdir, float* rgb);
 /* sort contract: stable ascending by key, ties by input position */

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
         if (COUNT) n_eval += done ? 0u : 1u;
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
-        const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core(-sigma));
+        const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core_fast(-sigma));
         const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
         if (acc) {
           if (COUNT) ++n_blend;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         if (COUNT) n_eval += done ? 0u : 1u;
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
-        const float gauss = rgs_exp_core(-sigma);
+        const float gauss = rgs_exp_core_fast(-sigma);
         const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
         const bool pass = need && !(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0));
         const float nT = T * (1.0f - alpha);
@@ -329,48 +329,50 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         done = done || stop;
         const uint64_t m = __ballot(contrib);
         if (m == 0) continue;
+        if (COUNT) n_contrib += contrib ? 1u : 0u;
+        // The gradient terms are evaluated on every lane (no divergent
+        // branch); lanes that do not contribute get vis = 0 and v_alpha = 0,
+        // which zeroes every term they add and leaves po and T unchanged.
+        // Their other intermediates are finite or masked by these selects.
         float val[12];
+        const float col[3] = {a1.z, a1.w, a2.x};
+        const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
+        const float vis = contrib ? alpha * T : 0.0f;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) val[k] = 0.0f;
-        if (contrib) {
-          if (COUNT) ++n_contrib;
-          const float col[3] = {a1.z, a1.w, a2.x};
-          const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
-          const float vis = alpha * T;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
-            po[c] = po[c] + vis * cl[c];
-          }
-          // 1/(1-alpha) with the hardware reciprocal (1 ulp); gradients are
-          // compared within a relative tolerance, not bit for bit.
-          const float ra = __builtin_amdgcn_rcpf(1.0f - alpha);
-          float tv[3];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
-          const float v_alpha = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
-          const float v_sigma = -alpha * v_alpha;
-          const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
-          const float vx = v_sigma * (ca * dx + cb * dy);
-          const float vy = v_sigma * (cb * dx + cc * dy);
-          if (a1.y * gauss <= RGS_F(0.999)) {
-            val[0] = vx;
-            val[1] = vy;
-            val[2] = 0.5f * v_sigma * dx * dx;
-            val[3] = v_sigma * dx * dy;
-            val[4] = 0.5f * v_sigma * dy * dy;
-            val[8] = v_alpha * gauss;
-            if (REFINE) {
-              const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
-              const float lx = vx * Wf, ly = vy * Hf;
-              val[9] = rgs_sqrt(lx * lx + ly * ly) / fa;
-            }
-            val[10] = fabsf(vx);
-            val[11] = fabsf(vy);
-          }
-          if (!REFINE) val[9] = 1.0f;  // coverage count rides in the refine slot
-          T = nT;
+        for (int c = 0; c < 3; ++c) {
+          val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
+          po[c] = po[c] + vis * cl[c];
         }
+        // 1/(1-alpha) with the hardware reciprocal (1 ulp); gradients are
+        // compared within a relative tolerance, not bit for bit.
+        const float ra = __builtin_amdgcn_rcpf(1.0f - alpha);
+        float tv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
+        const float v_alpha_all = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
+        // the alpha clamp gates the geometric terms (rasterize_backwards.wgsl:129-143)
+        const bool geo = contrib && a1.y * gauss <= RGS_F(0.999);
+        const float v_alpha = geo ? v_alpha_all : 0.0f;
+        const float v_sigma = -(geo ? alpha : 0.0f) * v_alpha;
+        const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
+        const float vx = v_sigma * (ca * dx + cb * dy);
+        const float vy = v_sigma * (cb * dx + cc * dy);
+        val[0] = vx;
+        val[1] = vy;
+        val[2] = 0.5f * v_sigma * dx * dx;
+        val[3] = v_sigma * dx * dy;
+        val[4] = 0.5f * v_sigma * dy * dy;
+        val[8] = v_alpha * (geo ? gauss : 0.0f);
+        if (REFINE) {
+          const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
+          const float lx = vx * Wf, ly = vy * Hf;
+          val[9] = rgs_sqrt(lx * lx + ly * ly) / fa;
+        } else {
+          val[9] = contrib ? 1.0f : 0.0f;  // coverage count rides in the refine slot
+        }
+        val[10] = fabsf(vx);
+        val[11] = fabsf(vy);
+        T = contrib ? nT : T;
         float tot[3];
         wave_reduce12(val, tot);
         if ((lane & 15) == 15) {

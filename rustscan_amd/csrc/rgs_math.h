@@ -81,6 +81,26 @@ RGS_HD float rgs_exp_core(float x) {
   return p * rgs_u2f((uint32_t)((int)k + 127) << 23);
 }
 
+// rgs_exp_core without the argument clamps, scaled with ldexp: equals
+// rgs_exp(x) bit for bit for every x in (-87, 88] (there 2^k is a normal
+// scale and p * 2^k is exact).  Outside that range, and for NaN, it returns
+// an arbitrary value (possibly inf or NaN) that the caller must discard; the
+// compositing kernels evaluate it on lanes whose result they then mask off.
+RGS_HD float rgs_exp_core_fast(float x) {
+  const float k = rgs_rint(x * 1.44269504088896341f);
+  float r = rgs_fma(-k, 0.693145751953125f, x);
+  r = rgs_fma(-k, 1.428606765330187045e-06f, r);
+  float p = 1.9841269841269841e-04f;
+  p = rgs_fma(p, r, 1.3888888888888889e-03f);
+  p = rgs_fma(p, r, 8.3333333333333333e-03f);
+  p = rgs_fma(p, r, 4.1666666666666667e-02f);
+  p = rgs_fma(p, r, 1.6666666666666667e-01f);
+  p = rgs_fma(p, r, 0.5f);
+  p = rgs_fma(p, r, 1.0f);
+  p = rgs_fma(p, r, 1.0f);
+  return __builtin_ldexpf(p, (int)k);
+}
+
 RGS_HD float rgs_log(float x) {
   if (x != x) return x;
   if (x <= 0.0f) return rgs_u2f(0xff800000u);

@@ -71,3 +71,18 @@ def test_exp_core_equals_exp(orc):
     rng = np.random.default_rng(13)
     y = rng.uniform(-86.99, 88.0, 2_000_000).astype(np.float32)
     np.testing.assert_array_equal(orc.math_exp_core(y), orc.math_exp(y))
+
+
+def test_exp_core_fast_equals_exp(orc):
+    """The clamp-free, ldexp-scaled exp of the compositing kernels equals
+    rgs_exp bit for bit on (-87, 88]; the kernels discard its value elsewhere."""
+    b0 = np.float32(-0.0).view(np.uint32)
+    b1 = np.float32(-5.6).view(np.uint32)
+    x = np.arange(b0, b1 + 1, 53, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    np.testing.assert_array_equal(orc.math_exp_core_fast(x), orc.math_exp(x))
+    rng = np.random.default_rng(14)
+    y = rng.uniform(-86.99, 88.0, 2_000_000).astype(np.float32)
+    np.testing.assert_array_equal(orc.math_exp_core_fast(y), orc.math_exp(y))
+    # edges of the range: the largest argument and arguments just above -87
+    e = np.array([88.0, np.nextafter(np.float32(-87.0), np.float32(0.0)), -86.5, 0.0, -0.0], np.float32)
+    np.testing.assert_array_equal(orc.math_exp_core_fast(e), orc.math_exp(e))
