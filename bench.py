@@ -76,6 +76,49 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a))
 
 
+def run_config3(ctx, opt, init, gt, steps: int = 200) -> dict:
+    """The reference's training semantics: one 1280x720 view and one Adam
+    update per step, cycling over 32 ring cameras, targets rendered from the
+    ground-truth scene; steps/s and the L1 loss at the start and the end."""
+    import torch
+
+    from rustscan_amd import scene
+    from rustscan_amd.rasterizer import DeviceSplats, Trainer, render_forward
+
+    cams = scene.config3_cameras()
+    g = DeviceSplats(ctx, gt)
+    targets = []
+    for c in cams:
+        targets.append(render_forward(ctx, g, [c])[0][..., :3].contiguous())
+    g.close()
+    splats = DeviceSplats(ctx, init)
+    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0)
+    first = [trainer.train_step(splats, [cams[i]], [targets[i]], batched=True, graph=True, read_loss=True)[0]
+             for i in range(8)]
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(steps):
+        i = it % len(cams)
+        trainer.train_step(splats, [cams[i]], [targets[i]], batched=True, graph=True)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    last = [trainer.train_step(splats, [cams[i]], [targets[i]], batched=True, graph=True, read_loss=True)[0]
+            for i in range(8)]
+    splats.close()
+    out = {
+        "workload": "1,048,576 Gaussians SH3, 32 ring views 1280x720, one view + one Adam update per step",
+        "steps": steps,
+        "steps_per_s": round(1.0 / dt, 2),
+        "ms_per_step": round(1e3 * dt, 3),
+        "l1_first_8_views": round(float(np.mean(first)), 5),
+        "l1_after_training": round(float(np.mean(last)), 5),
+        "skipped_steps": ctx.skipped_steps(),
+    }
+    log(f"config3: {out}")
+    return out
+
+
 def run_config5(args, ctx, opt) -> dict:
     """Large batched training step: 4,194,304 Gaussians (1% with scale x20),
     16 views at 1920x1080, uniform random targets; steps/s, intersections and
@@ -130,6 +173,9 @@ def main() -> int:
                     help="also measure the large configuration (4,194,304 Gaussians with a heavy scale tail, "
                          "16 views at 1920x1080 per batched step)")
     ap.add_argument("--config5-steps", type=int, default=3)
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip the single-view training loop (32 views at 1280x720, one view and one Adam "
+                         "update per step, 200 steps)")
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--batch-depth", type=int, default=256)
     args = ap.parse_args()
@@ -389,6 +435,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_tiled_renderer": cpu_render,
+        "config3": None if args.no_config3 else run_config3(ctx, opt, init, gt),
         "config5": run_config5(args, ctx, opt) if args.config5 else None,
         "device": dev_name,
     }

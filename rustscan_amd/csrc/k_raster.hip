@@ -334,7 +334,11 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         // branch); lanes that do not contribute get vis = 0 and v_alpha = 0,
         // which zeroes every term they add and leaves po and T unchanged.
         // Their other intermediates are finite or masked by these selects.
+        // The terms are fused into FMAs (they feed no compositing decision;
+        // gradients are compared within a relative tolerance).
         float val[12];
+        {
+#pragma clang fp contract(fast)
         const float col[3] = {a1.z, a1.w, a2.x};
         const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
         const float vis = contrib ? alpha * T : 0.0f;
@@ -372,6 +376,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         }
         val[10] = fabsf(vx);
         val[11] = fabsf(vy);
+        }
         T = contrib ? nT : T;
         float tot[3];
         wave_reduce12(val, tot);
