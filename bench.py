@@ -328,6 +328,12 @@ def main() -> int:
         if "raster_fwd" in per_launch and frags:
             stage_summary["raster_fwd"]["blended_fragments_per_s"] = round(
                 frags.get("fwd_blended", 0) / (per_launch["raster_fwd"] * 1e-3), 1)
+            # share of the 64 lanes of a (wave, splat) iteration whose pixel is still being composited
+            stage_summary["raster_fwd"]["active_lane_fraction"] = round(
+                frags.get("fwd_evaluated", 0) / max(1, 64 * frags.get("fwd_wave_iters", 0)), 4)
+        if "raster_bwd" in per_launch and frags:
+            stage_summary["raster_bwd"]["active_lane_fraction"] = round(
+                frags.get("bwd_evaluated", 0) / max(1, 64 * frags.get("bwd_wave_iters", 0)), 4)
         if "raster_bwd" in per_launch and frags:
             stage_summary["raster_bwd"]["contributing_fragments_per_s"] = round(
                 frags.get("bwd_contributing", 0) / (per_launch["raster_bwd"] * 1e-3), 1)

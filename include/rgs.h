@@ -210,10 +210,13 @@ int rgs_debug_get(rgs_ctx* ctx, int what, int view, void* dst, int64_t* count);
  * (RGS_DEBUG_RASTER_GRADS); costs one extra copy per backward.
  * flags bit 1: count compositing fragments in eager launches. */
 int rgs_set_debug(rgs_ctx* ctx, int flags);
-/* Fragment counts since the last reset (bit 1 of rgs_set_debug): [0] forward
- * (pixel, splat) pairs evaluated, [1] forward pairs blended, [2] backward pairs
- * evaluated, [3] backward pairs contributing a gradient.  Synchronous. */
-int rgs_get_fragment_counts(rgs_ctx* ctx, int64_t* out4, int reset);
+/* Fragment counts since the last reset (bit 1 of rgs_set_debug), 8 entries:
+ * [0] forward (pixel, splat) pairs evaluated, [1] forward pairs blended,
+ * [2] backward pairs evaluated, [3] backward pairs contributing a gradient,
+ * [4] forward (wave, splat) iterations, [5] of those where some lane
+ * evaluates exp, [6] backward (wave, splat) iterations, [7] of those with a
+ * contributing lane.  Synchronous. */
+int rgs_get_fragment_counts(rgs_ctx* ctx, int64_t* out8, int reset);
 
 /* ---- per-stage timing (HIP events on the context stream) ----------------
  * When enabled, eager (non-graph) launches record events around each stage;

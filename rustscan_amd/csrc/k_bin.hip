@@ -74,9 +74,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
   block_tiles<kRadixTile>(n, &t0, &t1);
+  const uint32_t begin = t0 * kRadixTile;
   const uint32_t end = t1 * kRadixTile < n ? t1 * kRadixTile : n;
-  for (uint32_t i = t0 * kRadixTile + threadIdx.x; i < end; i += kSortThreads)
-    atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
+  // four keys per thread per step (vector loads), several steps in flight;
+  // scalar loads when the range does not start 16-byte aligned
+  const bool aligned = ((reinterpret_cast<uintptr_t>(keys) + 4ull * begin) & 15u) == 0;
+  const uint32_t vend = aligned ? begin + ((end > begin ? end - begin : 0u) & ~3u) : begin;
+  const uint4* kv = reinterpret_cast<const uint4*>(keys);
+#pragma unroll 4
+  for (uint32_t q = begin / 4 + threadIdx.x; q < vend / 4; q += kSortThreads) {
+    const uint4 k = kv[q];
+    atomicAdd(&h[w][(k.x >> shift) & 255u], 1u);
+    atomicAdd(&h[w][(k.y >> shift) & 255u], 1u);
+    atomicAdd(&h[w][(k.z >> shift) & 255u], 1u);
+    atomicAdd(&h[w][(k.w >> shift) & 255u], 1u);
+  }
+  for (uint32_t i = vend + threadIdx.x; i < end; i += kSortThreads) atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t d = threadIdx.x;
   hist[(size_t)d * gridDim.x + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !inside;
-    uint32_t n_eval = 0, n_blend = 0;
+    uint32_t n_eval = 0, n_blend = 0, n_iter = 0, n_iter_work = 0;
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
@@ -177,9 +177,13 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
         }
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
-        if (COUNT) n_eval += done ? 0u : 1u;
+        if (COUNT) {
+          n_eval += done ? 0u : 1u;
+          ++n_iter;
+        }
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
+        if (COUNT) ++n_iter_work;
         const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core_fast(-sigma));
         const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
         if (acc) {
@@ -222,6 +226,8 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
       if ((tid & 63) == 0) {
         atomicAdd(&A.frag[0], e);
         atomicAdd(&A.frag[1], bl);
+        atomicAdd(&A.frag[4], (unsigned long long)n_iter);
+        atomicAdd(&A.frag[5], (unsigned long long)n_iter_work);
       }
     }
   }
@@ -289,7 +295,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
     float T = 1.0f;
     float po[3] = {0.f, 0.f, 0.f};
     bool done = !inside;
-    uint32_t n_eval = 0, n_contrib = 0;
+    uint32_t n_eval = 0, n_contrib = 0, n_iter = 0, n_iter_work = 0;
     const float Wf = (float)V.W, Hf = (float)V.H;
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
@@ -317,7 +323,10 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         }
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
-        if (COUNT) n_eval += done ? 0u : 1u;
+        if (COUNT) {
+          n_eval += done ? 0u : 1u;
+          ++n_iter;
+        }
         const bool need = !done && !(sigma > a2.y);
         if (!__any(need)) continue;
         const float gauss = rgs_exp_core_fast(-sigma);
@@ -329,7 +338,10 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         done = done || stop;
         const uint64_t m = __ballot(contrib);
         if (m == 0) continue;
-        if (COUNT) n_contrib += contrib ? 1u : 0u;
+        if (COUNT) {
+          n_contrib += contrib ? 1u : 0u;
+          ++n_iter_work;
+        }
         // The gradient terms are evaluated on every lane (no divergent
         // branch); lanes that do not contribute get vis = 0 and v_alpha = 0,
         // which zeroes every term they add and leaves po and T unchanged.
@@ -408,6 +420,8 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
       if ((tid & 63) == 0) {
         atomicAdd(&A.frag[2], e);
         atomicAdd(&A.frag[3], cb2);
+        atomicAdd(&A.frag[6], (unsigned long long)n_iter);
+        atomicAdd(&A.frag[7], (unsigned long long)n_iter_work);
       }
     }
   }

@@ -152,7 +152,7 @@ struct rgs_ctx {
   std::vector<Pending> pending;
   double stage_ms[RGS_STAGE_COUNT] = {};
   int64_t stage_calls[RGS_STAGE_COUNT] = {};
-  // fragment counters (device, 4 x u64) when enabled
+  // fragment counters (device, kFragCounters x u64) when enabled
   DevBuf frag;
   bool count_frags = false;
 };
@@ -1308,23 +1308,23 @@ int rgs_set_debug(rgs_ctx* c, int flags) {
   c->keep_raster_grads = (flags & 1) != 0;
   c->count_frags = (flags & 2) != 0;
   if (c->count_frags && !c->frag.p) {
-    if (ensure(c, c->frag, 4 * sizeof(unsigned long long))) return RGS_ERR_HIP;
-    HIPCHK(c, hipMemsetAsync(c->frag.p, 0, 4 * sizeof(unsigned long long), c->stream));
+    if (ensure(c, c->frag, kFragCounters * sizeof(unsigned long long))) return RGS_ERR_HIP;
+    HIPCHK(c, hipMemsetAsync(c->frag.p, 0, kFragCounters * sizeof(unsigned long long), c->stream));
   }
   invalidate_graph(c);
   return RGS_OK;
 }
 
-int rgs_get_fragment_counts(rgs_ctx* c, int64_t* out4, int reset) {
-  if (!c || !out4) return RGS_ERR_INVALID;
+int rgs_get_fragment_counts(rgs_ctx* c, int64_t* out8, int reset) {
+  if (!c || !out8) return RGS_ERR_INVALID;
   if (!c->frag.p) {
-    for (int i = 0; i < 4; ++i) out4[i] = 0;
+    for (int i = 0; i < kFragCounters; ++i) out8[i] = 0;
     return RGS_OK;
   }
-  unsigned long long h[4];
+  unsigned long long h[kFragCounters];
   HIPCHK(c, hipMemcpyAsync(h, c->frag.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int i = 0; i < 4; ++i) out4[i] = (int64_t)h[i];
+  for (int i = 0; i < kFragCounters; ++i) out8[i] = (int64_t)h[i];
   if (reset) HIPCHK(c, hipMemsetAsync(c->frag.p, 0, sizeof(h), c->stream));
   return RGS_OK;
 }

@@ -85,9 +85,13 @@ struct RasterArgs {
   float* loss;      // per-view sums of |pred - target|, or null
   float* acc;       // backward: [instances][16]
   // optional fragment counters: forward [0] evaluated, [1] blended;
-  // backward [2] evaluated, [3] contributing
+  // backward [2] evaluated, [3] contributing; per-wave splat iterations:
+  // [4] forward, [5] forward with a lane needing exp, [6] backward,
+  // [7] backward with a contributing lane
   unsigned long long* frag;
 };  // floats per instance in the raster-gradient accumulator
+
+constexpr int kFragCounters = 8;
 
 inline int sh_coeffs(int degree) { return (degree + 1) * (degree + 1); }
 

@@ -126,9 +126,11 @@ class Context:
         self.check(self.lib.rgs_set_debug(self.h, flags))
 
     def fragment_counts(self, reset: bool = True) -> dict:
-        out = (c_int64 * 4)()
+        out = (c_int64 * 8)()
         self.check(self.lib.rgs_get_fragment_counts(self.h, out, 1 if reset else 0))
-        return dict(fwd_evaluated=out[0], fwd_blended=out[1], bwd_evaluated=out[2], bwd_contributing=out[3])
+        return dict(fwd_evaluated=out[0], fwd_blended=out[1], bwd_evaluated=out[2], bwd_contributing=out[3],
+                    fwd_wave_iters=out[4], fwd_wave_iters_exp=out[5], bwd_wave_iters=out[6],
+                    bwd_wave_iters_contrib=out[7])
 
 
 class DeviceSplats:
