@@ -42,8 +42,8 @@ FP32_PEAK_TFLOPS = 157.3
 # kernel of each timed stage in the batched step (the dominant one is looked up
 # in the HBM-traffic profile)
 STAGE_KERNELS = {
-    "project_fwd": "k_project_fwd<3>", "depth_sort": "k_radix_onesweep", "count_scan": "k_scan_down",
-    "emit": "k_emit", "tile_sort": "k_radix_onesweep", "tile_ranges": "k_tile_ranges",
+    "project_fwd": "k_project_fwd<3>", "depth_sort": "k_radix_scatter", "count_scan": "k_scan_down",
+    "emit": "k_emit", "tile_sort": "k_radix_scatter", "tile_ranges": "k_tile_ranges",
     "raster_fwd": "k_raster_fwd<{tw}, {bd}, true, false>", "raster_bwd": "k_raster_bwd<{tw}, {bd}, false, true, false>",
     "project_bwd": "k_project_bwd<3>", "adam": "k_adam",
 }

@@ -64,6 +64,17 @@ __device__ __forceinline__ void pixel_in_tile(int tid, uint32_t* ox, uint32_t* o
   *oy = (uint32_t)((w / BPR) * 8 + (l >> 3));
 }
 
+// Wave vote straight from the lane-mask builtin: applied to a compare, the
+// compare's own lane mask is the vote.  Every lane of the wave must be active.
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+constexpr float kInf = __builtin_huge_valf();
+
+// max(c, 0) for a splat colour c, returned as -0.0 where c < 0 (or NaN): the
+// value acts as 0 in every product and sum, and its sign bit carries the
+// colour clamp's gradient gate (c >= 0).  c = sum + 0.5 is never -0.0.
+__device__ __forceinline__ float clamp_colour(float c) { return c >= 0.0f ? c : -0.0f; }
+
 #define RGS_DPP(v, ctrl, rm, bm) \
   __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, (v)), ctrl, rm, bm, true))
 
@@ -146,10 +157,12 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
     const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    bool done = !inside;
+    // 0 while the pixel is being composited, +inf once it is done: the
+    // per-splat test is then one add and one compare (sigma + pen <= bound).
+    float pen = inside ? 0.0f : kInf;
     uint32_t n_eval = 0, n_blend = 0, n_iter = 0, n_iter_work = 0;
     for (uint32_t b = rs; b < re; b += BATCH) {
-      if (__syncthreads_count(!done) == 0) break;
+      if (__syncthreads_count(pen == 0.0f) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
       for (uint32_t j = tid; j < cnt; j += NT) {
         const uint32_t inst = A.isect_vals[b + j];
@@ -162,27 +175,18 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
       }
       __syncthreads();
       // Wave-uniform loop: every lane walks the batch until the whole wave is
-      // done; a finished lane only stops updating its pixel.  The next
-      // splat's LDS record is fetched one iteration ahead.
-      float4 n0 = s0[0], n1 = s1[0];
-      float2 n2 = s2[0];
-      for (uint32_t j = 0; j < cnt; ++j) {
-        if (__all(done)) break;
-        const float4 a0 = n0, a1 = n1;
-        const float2 a2 = n2;
-        if (j + 1 < cnt) {
-          n0 = s0[j + 1];
-          n1 = s1[j + 1];
-          n2 = s2[j + 1];
-        }
+      // done; a finished lane only stops updating its pixel.  Two splats per
+      // trip: the next splat's LDS record is fetched one splat ahead into
+      // alternating registers (a one-splat loop would copy it every trip).
+      auto composite = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
         if (COUNT) {
-          n_eval += done ? 0u : 1u;
+          n_eval += pen == 0.0f ? 1u : 0u;
           ++n_iter;
         }
-        const bool need = !done && !(sigma > a2.y);
-        if (!__any(need)) continue;
+        const bool need = sigma + pen <= a2.y;
+        if (!wave_any(need)) return;
         if (COUNT) ++n_iter_work;
         const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core_fast(-sigma));
         const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
@@ -194,8 +198,23 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
           cb = cb + a2.x * vis;
           T = T * (1.0f - alpha);
           sv[j] = 1u;
-          if (T < RGS_F(1e-4)) done = true;
+          pen = T < RGS_F(1e-4) ? kInf : pen;
         }
+      };
+      float4 p0 = s0[0], p1 = s1[0];
+      float2 p2 = s2[0];
+      for (uint32_t j = 0; j < cnt; j += 2) {
+        const uint32_t j1 = j + 1 < cnt ? j + 1 : j;
+        const float4 q0 = s0[j1], q1 = s1[j1];
+        const float2 q2 = s2[j1];
+        if (!wave_any(pen == 0.0f)) break;
+        composite(p0, p1, p2, j);
+        if (j + 1 >= cnt || !wave_any(pen == 0.0f)) break;
+        const uint32_t j2 = j + 2 < cnt ? j + 2 : j1;
+        p0 = s0[j2];
+        p1 = s1[j2];
+        p2 = s2[j2];
+        composite(q0, q1, q2, j + 1);
       }
       __syncthreads();
       if (A.visible)
@@ -242,8 +261,10 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
   constexpr int NA = 13;
   constexpr int SA = 16;  // LDS stride per splat
   __shared__ float4 s0[BATCH];  // x, y, a/2, b
-  __shared__ float4 s1[BATCH];  // c/2, opacity, r, g (unclamped)
-  __shared__ float2 s2[BATCH];  // b (unclamped), exp-skip bound
+  // colours as stored by clamp_colour(): max(c, 0), with the sign bit set
+  // where c < 0 (the colour clamp's gradient gate)
+  __shared__ float4 s1[BATCH];  // c/2, opacity, r, g
+  __shared__ float2 s2[BATCH];  // b, exp-skip bound
   __shared__ uint32_t si[BATCH];
   __shared__ float sacc[BATCH * SA];
   __shared__ uint32_t s_item;
@@ -294,50 +315,42 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
     }
     float T = 1.0f;
     float po[3] = {0.f, 0.f, 0.f};
-    bool done = !inside;
+    float pen = inside ? 0.0f : kInf;  // +inf once the pixel is done
     uint32_t n_eval = 0, n_contrib = 0, n_iter = 0, n_iter_work = 0;
     const float Wf = (float)V.W, Hf = (float)V.H;
     for (uint32_t b = rs; b < re; b += BATCH) {
-      if (__syncthreads_count(!done) == 0) break;
+      if (__syncthreads_count(pen == 0.0f) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
       for (uint32_t j = tid; j < cnt; j += NT) {
         const uint32_t inst = A.isect_vals[b + j];
         const ProjSplat q = A.proj[inst];
         s0[j] = q.p0;
-        s1[j] = q.p1;
-        s2[j] = make_float2(q.p2.x, q.p2.y + kSkipMargin);
+        s1[j] = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
+        s2[j] = make_float2(clamp_colour(q.p2.x), q.p2.y + kSkipMargin);
         si[j] = inst;
       }
       for (uint32_t k = tid; k < cnt * SA; k += NT) sacc[k] = 0.0f;
       __syncthreads();
-      float4 n0 = s0[0], n1 = s1[0];
-      float2 n2 = s2[0];
-      for (uint32_t j = 0; j < cnt; ++j) {
-        if (__all(done)) break;
-        const float4 a0 = n0, a1 = n1;
-        const float2 a2 = n2;
-        if (j + 1 < cnt) {
-          n0 = s0[j + 1];
-          n1 = s1[j + 1];
-          n2 = s2[j + 1];
-        }
+      // Two splats per trip, the next record fetched one splat ahead into
+      // alternating registers, as in the forward.
+      auto accumulate = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
         if (COUNT) {
-          n_eval += done ? 0u : 1u;
+          n_eval += pen == 0.0f ? 1u : 0u;
           ++n_iter;
         }
-        const bool need = !done && !(sigma > a2.y);
-        if (!__any(need)) continue;
+        const bool need = sigma + pen <= a2.y;
+        if (!wave_any(need)) return;
         const float gauss = rgs_exp_core_fast(-sigma);
         const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
         const bool pass = need && !(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0));
         const float nT = T * (1.0f - alpha);
         const bool stop = pass && nT <= RGS_F(1e-4);
         const bool contrib = pass && !stop;
-        done = done || stop;
-        const uint64_t m = __ballot(contrib);
-        if (m == 0) continue;
+        pen = stop ? kInf : pen;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(contrib);
+        if (m == 0) return;
         if (COUNT) {
           n_contrib += contrib ? 1u : 0u;
           ++n_iter_work;
@@ -351,12 +364,11 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         float val[12];
         {
 #pragma clang fp contract(fast)
-        const float col[3] = {a1.z, a1.w, a2.x};
-        const float cl[3] = {fmaxf_(a1.z, 0.0f), fmaxf_(a1.w, 0.0f), fmaxf_(a2.x, 0.0f)};
+        const float cl[3] = {a1.z, a1.w, a2.x};
         const float vis = contrib ? alpha * T : 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          val[5 + c] = col[c] >= 0.0f ? vis * vo[c] : 0.0f;
+          val[5 + c] = __float_as_int(cl[c]) >= 0 ? vis * vo[c] : 0.0f;
           po[c] = po[c] + vis * cl[c];
         }
         // 1/(1-alpha) with the hardware reciprocal (1 ulp); gradients are
@@ -398,6 +410,21 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
           for (int a = 0; a < 3; ++a) atomicAdd(dst + slot[a], tot[a]);
           if (REFINE && lane == 63) atomicAdd(dst + 12, (float)__popcll(m));
         }
+      };
+      float4 p0 = s0[0], p1 = s1[0];
+      float2 p2 = s2[0];
+      for (uint32_t j = 0; j < cnt; j += 2) {
+        const uint32_t j1 = j + 1 < cnt ? j + 1 : j;
+        const float4 q0 = s0[j1], q1 = s1[j1];
+        const float2 q2 = s2[j1];
+        if (!wave_any(pen == 0.0f)) break;
+        accumulate(p0, p1, p2, j);
+        if (j + 1 >= cnt || !wave_any(pen == 0.0f)) break;
+        const uint32_t j2 = j + 2 < cnt ? j + 2 : j1;
+        p0 = s0[j2];
+        p1 = s1[j2];
+        p2 = s2[j2];
+        accumulate(q0, q1, q2, j + 1);
       }
       __syncthreads();
       // Flush: 16 consecutive threads per splat, one float each, so each
