@@ -296,16 +296,22 @@ def main() -> int:
                 per_launch[name] = ms / calls
         # algorithmic bytes (HBM-bound stages) / flops (compositing) per launch
         sort_passes_depth = 4
-        sort_passes_tile = (max(1, int(np.ceil(np.log2(max(2, sum(c[2] for c in counts)))))) + 7) // 8
+        tile_bits = max(1, int(np.ceil(np.log2(max(2, sum(c[2] for c in counts))))))
+        sort_passes_tile = (tile_bits + 7) // 8
+        inst_bits = int(np.ceil(np.log2(max(2, n * len(counts)))))
+        # two passes with a packed middle array (keys of 9..16 bits): key+value read, one word written,
+        # one word read, key+value written, plus one key word read per pass by the histogram
+        tile_sort_bytes = (32 * M if 8 < tile_bits <= 16 and tile_bits - 8 + inst_bits <= 32
+                           else 20 * M * sort_passes_tile)
         T = sum(c[2] for c in counts)
         work = {
             # parameters read once per Gaussian; ProjSplat + key + value + count written per instance
             "project_fwd": ("hbm", (11 + 3 * k) * 4 * n + (64 + 12) * n * nv),
-            "depth_sort": ("hbm", 16 * n * nv * sort_passes_depth + 4 * n * nv),
+            "depth_sort": ("hbm", 20 * n * nv * sort_passes_depth + 4 * n * nv),
             "count_scan": ("hbm", 12 * n * nv),
             # ProjSplat + order + offset read per visible instance; (key, value) written per intersection
             "emit": ("hbm", 72 * V + 8 * M),
-            "tile_sort": ("hbm", 16 * M * sort_passes_tile + 4 * M),
+            "tile_sort": ("hbm", tile_sort_bytes),
             "tile_ranges": ("hbm", 4 * M + 8 * T),
             # 28 FP32 operations per evaluated fragment (sigma 11, exp 1, alpha 2, tests 2, blend 9,
             # transmittance 3); about 95 per contributing fragment in the backward

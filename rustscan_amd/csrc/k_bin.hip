@@ -120,15 +120,27 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_spine(uint32_t* __restri
   if (threadIdx.x == 0) tot[blockIdx.x] = total;
 }
 
+// Scatter modes.  kScatterPairs: (key, value) in and out.  The two passes of
+// a packed two-pass sort (keys of 9..16 bits, value < 2^ib, key_bits - 8 +
+// ib <= 32): kScatterPackOut sorts (key, value) pairs on the low 8 key bits
+// and writes one word per item, (key >> 8) << ib | value; kScatterPackIn
+// sorts those words on bits [ib, ib + 8) and writes (key, value) pairs again,
+// taking the low key byte from the item's position (the first pass left the
+// words grouped by it; tot1 holds its digit totals).  The middle array then
+// moves 4 bytes per item instead of 8.
+enum ScatterMode : int { kScatterPairs = 0, kScatterPackOut = 1, kScatterPackIn = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev, uint32_t n_host, uint32_t cap, int shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, const uint32_t* __restrict__ gsrc,
-    uint32_t* __restrict__ gout) {
+    uint32_t* __restrict__ gout, int ib, const uint32_t* __restrict__ tot1) {
   __shared__ uint32_t base[256];     // next global slot of each digit for this block
   __shared__ uint32_t gbase[256];    // global slot of local position 0 of each digit's run in the tile
   __shared__ uint32_t wh[4][256];    // per-wave digit counts, then per-wave local bases
   __shared__ uint32_t sk[kRadixTile], sv[kRadixTile];
+  __shared__ uint32_t lo_start[MODE == kScatterPackIn ? 257 : 1];  // first position of each low byte
   __shared__ uint32_t tmp[8];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   {
@@ -136,6 +148,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t db = block_excl_scan256(tot[threadIdx.x], tmp, &total);
     base[threadIdx.x] = db + hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
   }
+  if (MODE == kScatterPackIn) {
+    uint32_t total;
+    lo_start[threadIdx.x] = block_excl_scan256(tot1[threadIdx.x], tmp, &total);
+    if (threadIdx.x == 0) lo_start[256] = 0xffffffffu;
+  }
+  const uint32_t vmask = MODE == kScatterPairs ? 0u : (ib >= 32 ? 0xffffffffu : ((1u << ib) - 1u));
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
   block_tiles<kRadixTile>(n, &t0, &t1);
@@ -147,12 +165,29 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     //    in index order, with a running count per digit
     uint32_t key[kRadixRows], val[kRadixRows], loc[kRadixRows];
     const uint32_t wbase = t * kRadixTile + (uint32_t)w * (64 * kRadixRows);
+    // packed input: low key byte of position wbase (largest d with
+    // lo_start[d] <= wbase), then advanced per item
+    uint32_t lo_d = 0, lo_next = 0;
+    if (MODE == kScatterPackIn) {
+      uint32_t a = 0, b = 256;
+      while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (lo_start[mid] <= wbase) a = mid; else b = mid;
+      }
+      lo_d = a;
+      lo_next = lo_start[a + 1];
+    }
 #pragma unroll
     for (int r = 0; r < kRadixRows; ++r) {
       const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
       const bool ok = idx < n;
       key[r] = ok ? kin[idx] : 0u;
-      val[r] = ok ? (vin ? vin[idx] : idx) : 0u;
+      if (MODE == kScatterPackIn) {
+        while (lo_d < 255u && idx >= lo_next) lo_next = lo_start[++lo_d + 1];
+        val[r] = lo_d;
+      } else {
+        val[r] = ok ? (vin ? vin[idx] : idx) : 0u;
+      }
       const uint32_t d = (key[r] >> shift) & 255u;
       uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -203,9 +238,16 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       const uint32_t k = sk[i];
       const uint32_t dst = gbase[(k >> shift) & 255u] + i;
       const uint32_t v = sv[i];
-      kout[dst] = k;
-      vout[dst] = v;
-      if (gout) gout[dst] = gsrc[v];
+      if (MODE == kScatterPairs) {
+        kout[dst] = k;
+        vout[dst] = v;
+        if (gout) gout[dst] = gsrc[v];
+      } else if (MODE == kScatterPackOut) {
+        kout[dst] = ((k >> 8) << ib) | v;
+      } else {
+        kout[dst] = ((k >> ib) << 8) | v;
+        vout[dst] = k & vmask;
+      }
     }
     __syncthreads();
   }
@@ -536,13 +578,42 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
     const bool last = p + 1 == passes;
     k_radix_hist<<<G, kSortThreads, 0, s>>>(sk, n_dev, n_host, cap, 8 * p, hist);
     k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot);
-    k_radix_scatter<<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot,
-                                               last ? gsrc : nullptr, last ? gout : nullptr);
+    k_radix_scatter<kScatterPairs><<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot,
+                                                              last ? gsrc : nullptr, last ? gout : nullptr, 0,
+                                                              nullptr);
     sk = dk;
     sv = dv;
     where = (p % 2 == 0) ? 1 : 2;
   }
   return where;
+}
+
+// Stable sort of (key, value) pairs with keys of key_bits <= 16 and values
+// below 2^value_bits, result in (kout, vout); `mid` (cap words) holds the
+// packed words between the two passes.  Falls back to radix_sort_pairs
+// (result in kb/vb or kout/vout, as it reports) when the packing does not fit.
+// Returns 1 if the result is in (kb, vb), 2 if in (kout, vout).
+int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
+                            uint32_t* kout, uint32_t* vout, const uint32_t* n_dev, uint32_t n_host, uint32_t cap,
+                            int key_bits, int value_bits, uint32_t* scratch) {
+  if (key_bits <= 8 || key_bits > 16 || key_bits - 8 + value_bits > 32)
+    return radix_sort_pairs(s, kin, vin, kb, vb, kout, vout, n_dev, n_host, cap, key_bits, scratch, nullptr,
+                            nullptr);
+  const uint32_t G = grid_for<kRadixTile>(cap);
+  uint32_t* hist = scratch;
+  uint32_t* tot_a = scratch + 256 * (size_t)G;
+  uint32_t* tot_b = tot_a + 256;
+  uint32_t* mid = kb;
+  k_radix_hist<<<G, kSortThreads, 0, s>>>(kin, n_dev, n_host, cap, 0, hist);
+  k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot_a);
+  k_radix_scatter<kScatterPackOut><<<G, kSortThreads, 0, s>>>(kin, vin, mid, nullptr, n_dev, n_host, cap, 0, hist,
+                                                              tot_a, nullptr, nullptr, value_bits, nullptr);
+  k_radix_hist<<<G, kSortThreads, 0, s>>>(mid, n_dev, n_host, cap, value_bits, hist);
+  k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot_b);
+  k_radix_scatter<kScatterPackIn><<<G, kSortThreads, 0, s>>>(mid, nullptr, kout, vout, n_dev, n_host, cap,
+                                                             value_bits, hist, tot_b, nullptr, nullptr, value_bits,
+                                                             tot_a);
+  return 2;
 }
 
 // Exclusive (or inclusive) scan of in[gather[i]] (gather may be null) for

@@ -179,6 +179,11 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                      uint32_t* __restrict__ cnt, uint32_t* __restrict__ counters) {
   constexpr int NK = (DEG + 1) * (DEG + 1);
+  // per wave: the tile-test parameters of its 64 lanes, and the start
+  // markers of their bboxes in the wave's joint tile list
+  __shared__ float4 s_par[4][64 * 3];
+  __shared__ __align__(16) uint8_t s_mark[4][64 * 64];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gid < n;
   const uint32_t gl = live ? gid : 0u;  // lanes past n run the view loop with pass = false
@@ -202,7 +207,7 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
     const float mcy = ((V.vm[1] * mean[0] + V.vm[5] * mean[1]) + V.vm[9] * mean[2]) + V.vm[13];
     const float mcz = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
     bool pass = qok && !(mcz < RGS_F(0.01) || mcz > RGS_F(1e10));
-    uint32_t count = 0;
+    uint32_t count = 0, nb = 0;
     bool big = false;
     float m2x = 0.f, m2y = 0.f, a = 0.f, b = 0.f, c = 0.f, thr = 0.f;
     uint32_t bb[4] = {0u, 0u, 0u, 0u};
@@ -238,25 +243,77 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
           float ex5, ey5;
           bbox_extent(cv.c00, cv.c11, thr, &ex5, &ey5);
           tile_bbox(m2x, m2y, ex5, ey5, V.tiles_x, V.tiles_y, tw, bb);
-          const uint32_t nb = (bb[3] - bb[1]) * (bb[2] - bb[0]);
-          uint64_t mask = 0;
+          nb = (bb[3] - bb[1]) * (bb[2] - bb[0]);
           big = nb > 64u;
-          if (!big) count = count_tiles(bb, tw, m2x, m2y, a, b, c, thr, &mask);
           // colour
           const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
           const float dd = (dx * dx + dy * dy) + dz * dz;
           const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
           float col[3];
           sh_color<DEG>(sh, dx * rs, dy * rs, dz * rs, col);
-          ProjSplat ps;
+          ProjSplat& ps = proj[inst];
           ps.p0 = make_float4(m2x, m2y, 0.5f * a, b);
           ps.p1 = make_float4(0.5f * c, opacity, col[0] + 0.5f, col[1] + 0.5f);
           ps.p2 = make_float4(col[2] + 0.5f, thr, pack_u16x2(bb[0], bb[1]), pack_u16x2(bb[2], bb[3]));
-          ps.p3 = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.0f, 0.0f);
-          proj[inst] = ps;
           keys[inst] = __float_as_uint(mcz);
         }
       }
+    }
+    // bboxes of at most 64 tiles: the wave tests its lanes' tiles together.
+    // The bboxes are laid end to end (exclusive scan of their sizes) and
+    // each lane tests one tile of that joint list per step, so the steps
+    // number total/64 instead of the largest bbox of the wave.  A lane finds
+    // the owner of its tile as the last bbox start at or before it (max-scan
+    // of the start markers) and reads the owner's parameters from LDS; each
+    // owner collects its bits from the step's ballot.  Tiles are visited in
+    // the reference's row-major bbox order: bit k of the mask is tile k.
+    const bool small = pass && !big;
+    const uint32_t nbs = small ? nb : 0u;
+    const uint32_t incl = wave_incl_add(nbs);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    const uint32_t excl = incl - nbs;
+    uint64_t mask = 0;
+    if (total) {
+      float4* par = s_par[wv];
+      uint8_t* mark = s_mark[wv];
+      const uint32_t bw = bb[2] - bb[0];
+      par[lane * 3 + 0] = make_float4(m2x, m2y, a, b);
+      par[lane * 3 + 1] = make_float4(c, thr, __uint_as_float(bb[0] | (bb[1] << 16)), __uint_as_float(bw));
+      par[lane * 3 + 2] = make_float4(__uint_as_float(excl), 1.0f / (float)(bw ? bw : 1u), 0.0f, 0.0f);
+      for (uint32_t z = lane * 16u; z < total; z += 1024u) *reinterpret_cast<uint4*>(mark + z) = make_uint4(0, 0, 0, 0);
+      wave_lds_fence();
+      if (nbs) mark[excl] = (uint8_t)(lane + 1u);
+      wave_lds_fence();
+      uint32_t carry = 0;
+      for (uint32_t c0 = 0; c0 < total; c0 += 64u) {
+        const uint32_t g = c0 + lane;
+        const bool valid = g < total;
+        uint32_t o = wave_incl_max(valid ? (uint32_t)mark[g] : 0u);
+        o = o > carry ? o : carry;
+        carry = (uint32_t)__builtin_amdgcn_readlane((int)o, 63);
+        const uint32_t ow = valid ? o - 1u : 0u;
+        const float4 q0 = par[ow * 3 + 0], q1 = par[ow * 3 + 1], q2 = par[ow * 3 + 2];
+        const uint32_t obb = __float_as_uint(q1.z);
+        uint32_t row, col;
+        bbox_rc(g - __float_as_uint(q2.x), __float_as_uint(q1.w), q2.y, &row, &col);
+        const bool hit = valid && tile_contributes((obb & 0xffffu) + col, (obb >> 16) + row, tw, q0.x, q0.y, q0.z,
+                                                   q0.w, q1.x, q1.y);
+        const uint64_t B = __ballot(hit);
+        const uint32_t lo = excl > c0 ? excl : c0;
+        const uint32_t hi = excl + nbs < c0 + 64u ? excl + nbs : c0 + 64u;
+        if (lo < hi) {
+          const uint32_t len = hi - lo;
+          const uint64_t bits = (B >> (lo - c0)) & (len == 64u ? ~0ull : ((1ull << len) - 1ull));
+          mask |= bits << (lo - excl);
+        }
+      }
+      wave_lds_fence();
+    }
+    if (small) {
+      count = (uint32_t)__popcll(mask);
+      proj[inst].p3 = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.0f, 0.0f);
+    } else if (pass) {
+      proj[inst].p3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // bboxes of more than 64 tiles: the wave counts them together
     uint64_t bigm = __ballot(big);

@@ -29,6 +29,9 @@ uint64_t radix_scratch_words(uint64_t cap);
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
                      uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout);
+int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
+                            uint32_t* kout, uint32_t* vout, const uint32_t* n_dev, uint32_t n_host, uint32_t cap,
+                            int key_bits, int value_bits, uint32_t* scratch);
 void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const uint32_t* n_dev, uint32_t n_host,
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev,
               uint32_t* scratch);
@@ -403,12 +406,14 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
                 vs.tw, nitems, (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
   }
   const int bits = (int)ceil_log2(tile_end);
+  // instance ids are below (v0 + nv) * n
+  const int inst_bits = (int)ceil_log2((uint64_t)(v0 + nv) * n);
   int wt;
   {
     StageTimer t(c, RGS_STAGE_TILE_SORT);
-    wt = radix_sort_pairs(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(), w.iv1.as<uint32_t>(),
-                          w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect, 0, (uint32_t)w.cap_isect,
-                          bits, w.scratch.as<uint32_t>(), nullptr, nullptr);
+    wt = radix_sort_pairs_packed(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(),
+                                 w.iv1.as<uint32_t>(), w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect,
+                                 0, (uint32_t)w.cap_isect, bits, inst_bits, w.scratch.as<uint32_t>());
   }
   uint32_t* skeys = wt == 1 ? w.ik1.as<uint32_t>() : w.ik0.as<uint32_t>();
   uint32_t* svals = wt == 1 ? w.iv1.as<uint32_t>() : w.iv0.as<uint32_t>();

@@ -156,6 +156,36 @@ __device__ __forceinline__ uint32_t wave_count_tiles(const uint32_t bb[4], int t
   return total;
 }
 
+// Inclusive scans over the 64 lanes of a wave (every lane active): DPP row
+// shifts within each 16-lane row, then row 0's and row 2's totals into rows 1
+// and 3 (row_bcast:15), then lane 31's into rows 2 and 3 (row_bcast:31).
+// Disabled rows and out-of-row sources read 0, the identity of both ops.
+#define RGS_DPP_U32(v, ctrl, rm) (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(v), ctrl, rm, 0xf, true)
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t v) {
+  v += RGS_DPP_U32(v, 0x111, 0xf);
+  v += RGS_DPP_U32(v, 0x112, 0xf);
+  v += RGS_DPP_U32(v, 0x114, 0xf);
+  v += RGS_DPP_U32(v, 0x118, 0xf);
+  v += RGS_DPP_U32(v, 0x142, 0xa);
+  v += RGS_DPP_U32(v, 0x143, 0xc);
+  return v;
+}
+__device__ __forceinline__ uint32_t umax_(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  v = umax_(v, RGS_DPP_U32(v, 0x111, 0xf));
+  v = umax_(v, RGS_DPP_U32(v, 0x112, 0xf));
+  v = umax_(v, RGS_DPP_U32(v, 0x114, 0xf));
+  v = umax_(v, RGS_DPP_U32(v, 0x118, 0xf));
+  v = umax_(v, RGS_DPP_U32(v, 0x142, 0xa));
+  v = umax_(v, RGS_DPP_U32(v, 0x143, 0xc));
+  return v;
+}
+#undef RGS_DPP_U32
+
+// Orders this wave's LDS accesses before and after it (the LDS executes one
+// wave's accesses in issue order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
 __device__ __forceinline__ int find_view(const ViewDev* __restrict__ views, int nv, uint32_t tile) {
   int v = 0;
   for (int i = 1; i < nv; ++i)
