@@ -314,7 +314,13 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
       vo[3] = (g[3] - ((V.bg[0] * g[0] + V.bg[1] * g[1]) + V.bg[2] * g[2])) * Tf;
     }
     float T = 1.0f;
-    float po[3] = {0.f, 0.f, 0.f};
+    // v_alpha of a splat (rasterize_backwards.wgsl:115-127) is
+    //   sum_c vo[c] * (T*clamp(col_c) + (po[c] - rgb[c]) / (1 - alpha)) + vo[3] / (1 - alpha)
+    // with po the colour composited up to and including it; it is evaluated as
+    //   T*Q + (P - R) / (1 - alpha),  Q = sum_c vo[c]*clamp(col_c),
+    //   P = sum_c vo[c]*po[c] (running), R = sum_c vo[c]*rgb[c] - vo[3].
+    const float R = ((vo[0] * rgbf[0] + vo[1] * rgbf[1]) + vo[2] * rgbf[2]) - vo[3];
+    float P = 0.0f;
     float pen = inside ? 0.0f : kInf;  // +inf once the pixel is done
     uint32_t n_eval = 0, n_contrib = 0, n_iter = 0, n_iter_work = 0;
     const float Wf = (float)V.W, Hf = (float)V.H;
@@ -357,39 +363,38 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         }
         // The gradient terms are evaluated on every lane (no divergent
         // branch); lanes that do not contribute get vis = 0 and v_alpha = 0,
-        // which zeroes every term they add and leaves po and T unchanged.
+        // which zeroes every term they add and leaves P and T unchanged.
         // Their other intermediates are finite or masked by these selects.
         // The terms are fused into FMAs (they feed no compositing decision;
-        // gradients are compared within a relative tolerance).
+        // gradients are compared within a relative tolerance).  Factors that
+        // are uniform over the splat are applied when the batch is flushed:
+        // the colour clamp's gate on v_rgb and the 1/2 of v_conic[0] and [2].
         float val[12];
         {
 #pragma clang fp contract(fast)
         const float cl[3] = {a1.z, a1.w, a2.x};
         const float vis = contrib ? alpha * T : 0.0f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          val[5 + c] = __float_as_int(cl[c]) >= 0 ? vis * vo[c] : 0.0f;
-          po[c] = po[c] + vis * cl[c];
-        }
+        for (int c = 0; c < 3; ++c) val[5 + c] = vis * vo[c];
+        const float Q = (cl[0] * vo[0] + cl[1] * vo[1]) + cl[2] * vo[2];
         // 1/(1-alpha) with the hardware reciprocal (1 ulp); gradients are
         // compared within a relative tolerance, not bit for bit.
         const float ra = __builtin_amdgcn_rcpf(1.0f - alpha);
-        float tv[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) tv[c] = T * cl[c] + (po[c] - rgbf[c]) * ra;
-        const float v_alpha_all = ((tv[0] * vo[0] + tv[1] * vo[1]) + tv[2] * vo[2]) + vo[3] * ra;
+        P = P + vis * Q;  // po includes this splat's own colour
+        const float v_alpha_all = T * Q + (P - R) * ra;
         // the alpha clamp gates the geometric terms (rasterize_backwards.wgsl:129-143)
         const bool geo = contrib && a1.y * gauss <= RGS_F(0.999);
         const float v_alpha = geo ? v_alpha_all : 0.0f;
-        const float v_sigma = -(geo ? alpha : 0.0f) * v_alpha;
+        const float v_sigma = -alpha * v_alpha;  // alpha is finite on every lane
         const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
-        const float vx = v_sigma * (ca * dx + cb * dy);
-        const float vy = v_sigma * (cb * dx + cc * dy);
+        const float sdx = v_sigma * dx, sdy = v_sigma * dy;
+        const float vx = ca * sdx + cb * sdy;
+        const float vy = cb * sdx + cc * sdy;
         val[0] = vx;
         val[1] = vy;
-        val[2] = 0.5f * v_sigma * dx * dx;
-        val[3] = v_sigma * dx * dy;
-        val[4] = 0.5f * v_sigma * dy * dy;
+        val[2] = sdx * dx;  // x 1/2 at the flush
+        val[3] = sdx * dy;
+        val[4] = sdy * dy;  // x 1/2 at the flush
         val[8] = v_alpha * (geo ? gauss : 0.0f);
         if (REFINE) {
           const float fa = fmaxf_(rgbf[3], RGS_F(1e-5));
@@ -433,7 +438,12 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
         const uint32_t j = e / SA, k = e % SA;
         if (k >= (uint32_t)NA) continue;
         if (sacc[j * SA + 12] == 0.0f) continue;
-        const float x = sacc[e];
+        float x = sacc[e];
+        if (k == 2 || k == 4) x *= 0.5f;
+        if (k >= 5 && k <= 7) {
+          const float col = k == 5 ? s1[j].z : (k == 6 ? s1[j].w : s2[j].x);
+          if (__float_as_int(col) < 0) x = 0.0f;  // clamped colour: no gradient
+        }
         if (x != 0.0f) atomicAdd(A.acc + (size_t)si[j] * kAcc + k, x);
       }
     }
