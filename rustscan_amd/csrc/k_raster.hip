@@ -32,7 +32,19 @@
 
 namespace rgs {
 
-__device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t* s_item, uint32_t& iter, uint32_t* item) {
+// A compositing workgroup covers at most 16x16 pixels: a 32x32 binning tile
+// is composited by four workgroups (its 16x16 sub-tiles), each walking the
+// tile's whole depth-sorted list and culling it per 8x8 block.  Work items
+// are (tile, sub-tile) pairs, the sub-tiles of a tile consecutive.
+template <int TW>
+struct RasterShape {
+  static constexpr int RW = TW > 16 ? 16 : TW;  // workgroup width in pixels
+  static constexpr int SUB = TW / RW;           // sub-tiles per tile side
+  static constexpr int NT = RW * RW;            // threads per workgroup
+};
+
+__device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t items, uint32_t* s_item, uint32_t& iter,
+                                          uint32_t* item) {
   if (A.queue) {
     if (threadIdx.x == 0) *s_item = atomicAdd(A.queue, 1u);
     __syncthreads();
@@ -42,7 +54,7 @@ __device__ __forceinline__ bool next_item(const RasterArgs& A, uint32_t* s_item,
     if (iter++) return false;
     *item = blockIdx.x;
   }
-  return *item < A.tile_count;
+  return *item < items;
 }
 
 template <bool MULTI>
@@ -133,26 +145,132 @@ __device__ __forceinline__ void wave_reduce12_slots(int slot[3]) {
   for (int a = 0; a < 3; ++a) slot[a] = (int)(t[a] * (1.0f / 64.0f) + 0.5f) - 1;
 }
 
+
+// Which 8x8 pixel blocks of a tile a staged splat can reach.  Bit
+// bx + by*(TW/8) is cleared only when every pixel centre of block (bx, by)
+// provably has sigma > bound, so the wave of that block would find no pixel
+// to composite and may skip the splat without changing any result.
+// Proof: the conic [[a, b], [b, c]] with a, c > 0 and det = ac - b^2 > 0 gives
+// 2*sigma(dx, dy) = c*(dy + b*dx/c)^2 + (det/c)*dx^2 >= (det/c)*dx^2, and
+// likewise >= (det/a)*dy^2.  Over the block, |dx| >= Xmin and |dy| >= Ymin
+// (rounding is monotone, so the float dx of every pixel lies between those of
+// the extreme pixel centres).  The float sigma of a pixel is within
+// 2.4e-7*(a*dx^2 + c*dy^2) of the exact one (four roundings per term,
+// |b*dx*dy| <= (a*dx^2 + c*dy^2)/2); E below bounds that with a 4x margin.
+// det is rounded down (det_lo <= det) and the lower bound scaled by 1 - 1e-5,
+// which covers the roundings of its own evaluation.  NaN or non-positive-
+// definite conics keep every bit.
+template <int TW>
+__device__ __forceinline__ uint32_t block_touch_mask(const float4 q0, float hc, float bound, uint32_t tox,
+                                                     uint32_t toy) {
+  constexpr int BPR = TW / 8;
+  constexpr uint32_t kAll = (BPR * BPR >= 32) ? 0xffffffffu : ((1u << (BPR * BPR)) - 1u);
+  const float a = 2.0f * q0.z, b = q0.w, c = 2.0f * hc;
+  const float det_lo = (a * c) * (1.0f - 1e-6f) - (b * b) * (1.0f + 1e-6f);
+  if (!(a > 0.0f && c > 0.0f && det_lo > 0.0f)) return kAll;
+  const float kx = 0.5f * det_lo / c, ky = 0.5f * det_lo / a;
+  float lx[BPR], ex[BPR], ly[BPR], ey[BPR];
+#pragma unroll
+  for (int i = 0; i < BPR; ++i) {
+    // dx = x - pcx over the pixel centres pcx = tox + 8i + {0.5 .. 7.5}
+    const float dlo = q0.x - ((float)(tox + 8 * i + 7) + 0.5f), dhi = q0.x - ((float)(tox + 8 * i) + 0.5f);
+    const float xmin = dlo > 0.0f ? dlo : (dhi < 0.0f ? -dhi : 0.0f);
+    const float xmax = fmaxf_(fabsf(dlo), fabsf(dhi));
+    lx[i] = kx * xmin * xmin;
+    ex[i] = a * xmax * xmax;
+    const float elo = q0.y - ((float)(toy + 8 * i + 7) + 0.5f), ehi = q0.y - ((float)(toy + 8 * i) + 0.5f);
+    const float ymin = elo > 0.0f ? elo : (ehi < 0.0f ? -ehi : 0.0f);
+    const float ymax = fmaxf_(fabsf(elo), fabsf(ehi));
+    ly[i] = ky * ymin * ymin;
+    ey[i] = c * ymax * ymax;
+  }
+  uint32_t bits = 0;
+#pragma unroll
+  for (int by = 0; by < BPR; ++by)
+#pragma unroll
+    for (int bx = 0; bx < BPR; ++bx) {
+      const float lb = fmaxf_(lx[bx], ly[by]) * (1.0f - 1e-5f);
+      const float e = 1e-6f * (ex[bx] + ey[by]);
+      if (!(lb - e > bound)) bits |= 1u << (bx + by * BPR);
+    }
+  return bits;
+}
+
+// Publishes the block masks of staged splats j = j0 + tid as 64-bit words
+// words[b][j / 64] (bit j % 64: splat j can reach block b).  Wave-uniform:
+// every lane of a wave calls it.
+template <int TW, int BATCH>
+__device__ __forceinline__ void publish_block_masks(uint64_t (*words)[BATCH / 64], uint32_t j0, uint32_t bits) {
+  constexpr int NB = (TW / 8) * (TW / 8);
+  const uint32_t wj = j0 + (threadIdx.x & ~63u);  // first splat of this wave
+  if (wj >= (uint32_t)BATCH) return;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64((bits >> b) & 1u);
+    if ((threadIdx.x & 63u) == 0) words[b][wj / 64] = m;
+  }
+}
+
+// Calls body(s0[j], s1[j], s2[j], j) for the staged splats j < cnt whose bit
+// is set in this wave's words, in increasing j, while alive() holds for some
+// lane.  Two splats per trip, the next record fetched one splat ahead into
+// alternating registers.
+template <int BATCH, typename Alive, typename Body>
+__device__ __forceinline__ void for_each_reaching(const uint64_t* words, uint32_t cnt, const float4* s0,
+                                                  const float4* s1, const float2* s2, Alive&& alive, Body&& body) {
+  const uint32_t nw = (cnt + 63) / 64;
+  for (uint32_t wi = 0; wi < nw; ++wi) {
+    const uint64_t wv = words[wi];
+    uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wv >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)wv);
+    if (m == 0) continue;
+    if (!wave_any(alive())) return;
+    uint32_t ja = wi * 64 + (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    float4 p0 = s0[ja], p1 = s1[ja];
+    float2 p2 = s2[ja];
+    while (true) {
+      const bool hb = m != 0;
+      const uint32_t jb = hb ? wi * 64 + (uint32_t)__builtin_ctzll(m) : ja;
+      m &= m - 1;
+      const float4 q0 = s0[jb], q1 = s1[jb];
+      const float2 q2 = s2[jb];
+      body(p0, p1, p2, ja);
+      if (!hb || !wave_any(alive())) break;
+      const bool ha = m != 0;
+      ja = ha ? wi * 64 + (uint32_t)__builtin_ctzll(m) : jb;
+      m &= m - 1;
+      p0 = s0[ja];
+      p1 = s1[ja];
+      p2 = s2[ja];
+      body(q0, q1, q2, jb);
+      if (!ha || !wave_any(alive())) break;
+    }
+  }
+}
+
 template <int TW, int BATCH, bool MULTI, bool COUNT>
-__global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
-  constexpr int NT = TW * TW;
+__global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A) {
+  constexpr int RW = RasterShape<TW>::RW, SUB = RasterShape<TW>::SUB, NT = RasterShape<TW>::NT;
   __shared__ float4 s0[BATCH];  // x, y, a/2, b
   __shared__ float4 s1[BATCH];  // c/2, opacity, max(r,0), max(g,0)
   __shared__ float2 s2[BATCH];  // max(b,0), exp-skip bound
   __shared__ uint32_t sg[BATCH];
   __shared__ uint32_t sv[BATCH];
+  __shared__ uint64_t swords[(RW / 8) * (RW / 8)][BATCH / 64];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
   uint32_t ox, oy;
-  pixel_in_tile<TW>(tid, &ox, &oy);
+  pixel_in_tile<RW>(tid, &ox, &oy);
   uint32_t iter = 0, item;
-  while (next_item(A, &s_item, iter, &item)) {
-    const uint32_t tile = A.tile_first + item;
+  while (next_item(A, A.tile_count * (SUB * SUB), &s_item, iter, &item)) {
+    const uint32_t tile = A.tile_first + item / (SUB * SUB), sub = item % (SUB * SUB);
     const int v = view_of<MULTI>(A, tile);
     const ViewDev& V = A.views[v];
     const uint32_t local = tile - V.tile_base;
     const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
-    const uint32_t px = tx * TW + ox, py = ty * TW + oy;
+    const uint32_t tox = tx * TW + (sub % SUB) * RW, toy = ty * TW + (sub / SUB) * RW;
+    const uint32_t px = tox + ox, py = toy + oy;
     const bool inside = px < V.W && py < V.H;
     const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
@@ -164,20 +282,25 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(pen == 0.0f) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
-      for (uint32_t j = tid; j < cnt; j += NT) {
-        const uint32_t inst = A.isect_vals[b + j];
-        const ProjSplat q = A.proj[inst];
-        s0[j] = q.p0;
-        s1[j] = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
-        s2[j] = make_float2(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin);
-        sg[j] = inst - (uint32_t)v * A.n;
-        sv[j] = 0u;
+      for (uint32_t j0 = 0; j0 < (uint32_t)BATCH; j0 += NT) {
+        const uint32_t j = j0 + tid;
+        uint32_t bits = 0;
+        if (j < cnt) {
+          const uint32_t inst = A.isect_vals[b + j];
+          const ProjSplat q = A.proj[inst];
+          s0[j] = q.p0;
+          s1[j] = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
+          s2[j] = make_float2(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin);
+          sg[j] = inst - (uint32_t)v * A.n;
+          sv[j] = 0u;
+          bits = block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy);
+        }
+        publish_block_masks<RW, BATCH>(swords, j0, bits);
       }
       __syncthreads();
-      // Wave-uniform loop: every lane walks the batch until the whole wave is
-      // done; a finished lane only stops updating its pixel.  Two splats per
-      // trip: the next splat's LDS record is fetched one splat ahead into
-      // alternating registers (a one-splat loop would copy it every trip).
+      // Wave-uniform loop: every lane walks the splats that can reach its
+      // wave's 8x8 block until the whole wave is done; a finished lane only
+      // stops updating its pixel.
       auto composite = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
@@ -201,21 +324,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
           pen = T < RGS_F(1e-4) ? kInf : pen;
         }
       };
-      float4 p0 = s0[0], p1 = s1[0];
-      float2 p2 = s2[0];
-      for (uint32_t j = 0; j < cnt; j += 2) {
-        const uint32_t j1 = j + 1 < cnt ? j + 1 : j;
-        const float4 q0 = s0[j1], q1 = s1[j1];
-        const float2 q2 = s2[j1];
-        if (!wave_any(pen == 0.0f)) break;
-        composite(p0, p1, p2, j);
-        if (j + 1 >= cnt || !wave_any(pen == 0.0f)) break;
-        const uint32_t j2 = j + 2 < cnt ? j + 2 : j1;
-        p0 = s0[j2];
-        p1 = s1[j2];
-        p2 = s2[j2];
-        composite(q0, q1, q2, j + 1);
-      }
+      for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, composite);
       __syncthreads();
       if (A.visible)
         for (uint32_t j = tid; j < cnt; j += NT)
@@ -253,8 +362,8 @@ __global__ __launch_bounds__(TW* TW) void k_raster_fwd(RasterArgs A) {
 }
 
 template <int TW, int BATCH, bool REFINE, bool MULTI, bool COUNT>
-__global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
-  constexpr int NT = TW * TW;
+__global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A) {
+  constexpr int RW = RasterShape<TW>::RW, SUB = RasterShape<TW>::SUB, NT = RasterShape<TW>::NT;
   // Accumulator slots per splat; the layout is that of the per-instance
   // accumulator: 0-1 v_xy, 2-4 v_conic, 5-7 v_rgb, 8 opacity term, 9 refine,
   // 10-11 |v_xy|, 12 coverage count.
@@ -267,24 +376,26 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
   __shared__ float2 s2[BATCH];  // b, exp-skip bound
   __shared__ uint32_t si[BATCH];
   __shared__ float sacc[BATCH * SA];
+  __shared__ uint64_t swords[(RW / 8) * (RW / 8)][BATCH / 64];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   uint32_t ox, oy;
-  pixel_in_tile<TW>(tid, &ox, &oy);
+  pixel_in_tile<RW>(tid, &ox, &oy);
   int slot[3];
   wave_reduce12_slots(slot);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
     if (!REFINE && slot[a] == 9) slot[a] = 12;  // coverage count rides in the refine slot
   uint32_t iter = 0, item;
-  while (next_item(A, &s_item, iter, &item)) {
-    const uint32_t tile = A.tile_first + item;
+  while (next_item(A, A.tile_count * (SUB * SUB), &s_item, iter, &item)) {
+    const uint32_t tile = A.tile_first + item / (SUB * SUB), sub = item % (SUB * SUB);
     const int v = view_of<MULTI>(A, tile);
     const ViewDev& V = A.views[v];
     const uint32_t local = tile - V.tile_base;
     const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
-    const uint32_t px = tx * TW + ox, py = ty * TW + oy;
+    const uint32_t tox = tx * TW + (sub % SUB) * RW, toy = ty * TW + (sub / SUB) * RW;
+    const uint32_t px = tox + ox, py = toy + oy;
     const bool inside = px < V.W && py < V.H;
     const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
     const uint32_t rs = A.ranges[2 * tile], re = A.ranges[2 * tile + 1];
@@ -327,18 +438,23 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
     for (uint32_t b = rs; b < re; b += BATCH) {
       if (__syncthreads_count(pen == 0.0f) == 0) break;
       const uint32_t cnt = min((uint32_t)BATCH, re - b);
-      for (uint32_t j = tid; j < cnt; j += NT) {
-        const uint32_t inst = A.isect_vals[b + j];
-        const ProjSplat q = A.proj[inst];
-        s0[j] = q.p0;
-        s1[j] = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
-        s2[j] = make_float2(clamp_colour(q.p2.x), q.p2.y + kSkipMargin);
-        si[j] = inst;
+      for (uint32_t j0 = 0; j0 < (uint32_t)BATCH; j0 += NT) {
+        const uint32_t j = j0 + tid;
+        uint32_t bits = 0;
+        if (j < cnt) {
+          const uint32_t inst = A.isect_vals[b + j];
+          const ProjSplat q = A.proj[inst];
+          s0[j] = q.p0;
+          s1[j] = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
+          s2[j] = make_float2(clamp_colour(q.p2.x), q.p2.y + kSkipMargin);
+          si[j] = inst;
+          bits = block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy);
+        }
+        publish_block_masks<RW, BATCH>(swords, j0, bits);
       }
       for (uint32_t k = tid; k < cnt * SA; k += NT) sacc[k] = 0.0f;
       __syncthreads();
-      // Two splats per trip, the next record fetched one splat ahead into
-      // alternating registers, as in the forward.
+      // Same walk as the forward: the splats that can reach the wave's block.
       auto accumulate = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
@@ -416,21 +532,7 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
           if (REFINE && lane == 63) atomicAdd(dst + 12, (float)__popcll(m));
         }
       };
-      float4 p0 = s0[0], p1 = s1[0];
-      float2 p2 = s2[0];
-      for (uint32_t j = 0; j < cnt; j += 2) {
-        const uint32_t j1 = j + 1 < cnt ? j + 1 : j;
-        const float4 q0 = s0[j1], q1 = s1[j1];
-        const float2 q2 = s2[j1];
-        if (!wave_any(pen == 0.0f)) break;
-        accumulate(p0, p1, p2, j);
-        if (j + 1 >= cnt || !wave_any(pen == 0.0f)) break;
-        const uint32_t j2 = j + 2 < cnt ? j + 2 : j1;
-        p0 = s0[j2];
-        p1 = s1[j2];
-        p2 = s2[j2];
-        accumulate(q0, q1, q2, j + 1);
-      }
+      for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, accumulate);
       __syncthreads();
       // Flush: 16 consecutive threads per splat, one float each, so each
       // atomic instruction covers whole 64-byte accumulator lines.
@@ -471,22 +573,22 @@ __global__ __launch_bounds__(TW* TW) void k_raster_bwd(RasterArgs A) {
 template <int TW, int BATCH>
 static void fwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid) {
   if (A.frag)
-    k_raster_fwd<TW, BATCH, true, true><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_fwd<TW, BATCH, true, true><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
   else if (A.nv > 1)
-    k_raster_fwd<TW, BATCH, true, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_fwd<TW, BATCH, true, false><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
   else
-    k_raster_fwd<TW, BATCH, false, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_fwd<TW, BATCH, false, false><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
 }
 template <int TW, int BATCH>
 static void bwd_launch(hipStream_t s, const RasterArgs& A, uint32_t grid, bool refine) {
   if (refine)
-    k_raster_bwd<TW, BATCH, true, true, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, true, true, false><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
   else if (A.frag)
-    k_raster_bwd<TW, BATCH, false, true, true><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, false, true, true><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
   else if (A.nv > 1)
-    k_raster_bwd<TW, BATCH, false, true, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, false, true, false><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
   else
-    k_raster_bwd<TW, BATCH, false, false, false><<<grid, TW * TW, 0, s>>>(A);
+    k_raster_bwd<TW, BATCH, false, false, false><<<grid, RasterShape<TW>::NT, 0, s>>>(A);
 }
 
 #define RGS_DISPATCH(TWV, BV, CALL) \
@@ -521,10 +623,14 @@ int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uin
   return 1;
 }
 
+// Workgroup size and work items (tile, sub-tile) of a tile width.
+int raster_threads(int tw) { return tw > 16 ? 256 : tw * tw; }
+uint32_t raster_items(int tw, uint32_t tiles) { return tw > 16 ? tiles * (uint32_t)((tw / 16) * (tw / 16)) : tiles; }
+
 // Occupancy-based persistent grid size for a configuration.
 int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus) {
   int per_cu = 0;
-  const int threads = tw * tw;
+  const int threads = raster_threads(tw);
 #define RGS_OCC(TWV, BV)                                                                                \
   if (tw == TWV && batch == BV) {                                                                       \
     if (bwd)                                                                                            \

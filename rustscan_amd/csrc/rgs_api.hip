@@ -48,6 +48,7 @@ void launch_set_u32(hipStream_t s, uint32_t* p, uint32_t v);
 int launch_raster_fwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid);
 int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uint32_t grid, bool refine);
 int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus);
+uint32_t raster_items(int tw, uint32_t tiles);
 void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, uint32_t npad, uint32_t n, int nplanes,
                  const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
                  uint32_t* step, int nviews, int check_loss);
@@ -356,6 +357,7 @@ int upload_views(rgs_ctx* c, const ViewSet& vs) {
 }
 
 uint32_t raster_grid(rgs_ctx* c, bool bwd, uint32_t tiles) {
+  tiles = raster_items((int)c->raster.tile_width, tiles);  // workgroup items: (tile, sub-tile)
   if (!c->raster.persistent) return tiles;
   const uint32_t g = (uint32_t)raster_persistent_grid((int)c->raster.tile_width, (int)c->raster.batch_depth, bwd,
                                                       c->num_cus);
