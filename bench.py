@@ -176,8 +176,8 @@ def main() -> int:
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the single-view training loop (32 views at 1280x720, one view and one Adam "
                          "update per step, 200 steps)")
-    ap.add_argument("--tile", type=int, default=16)
-    ap.add_argument("--batch-depth", type=int, default=256)
+    ap.add_argument("--tile", type=int, default=32)
+    ap.add_argument("--batch-depth", type=int, default=128)
     args = ap.parse_args()
 
     if args.gpus != 1:

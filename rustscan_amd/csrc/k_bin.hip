@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     lo_start[threadIdx.x] = block_excl_scan256(tot1[threadIdx.x], tmp, &total);
     if (threadIdx.x == 0) lo_start[256] = 0xffffffffu;
   }
-  const uint32_t vmask = MODE == kScatterPairs ? 0u : (ib >= 32 ? 0xffffffffu : ((1u << ib) - 1u));
+  const uint32_t vmask = (ib <= 0 || ib >= 32) ? 0xffffffffu : ((1u << ib) - 1u);
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
   block_tiles<kRadixTile>(n, &t0, &t1);
@@ -240,8 +240,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       const uint32_t v = sv[i];
       if (MODE == kScatterPairs) {
         kout[dst] = k;
-        vout[dst] = v;
-        if (gout) gout[dst] = gsrc[v];
+        if (ib > 0) {  // the value carries a count above bit ib; all ones = look it up in gsrc
+          const uint32_t inst = v & vmask, c = v >> ib;
+          vout[dst] = inst;
+          gout[dst] = c == (0xffffffffu >> ib) ? gsrc[inst] : c;
+        } else {
+          vout[dst] = v;
+          if (gout) gout[dst] = gsrc[v];
+        }
       } else if (MODE == kScatterPackOut) {
         kout[dst] = ((k >> 8) << ib) | v;
       } else {
@@ -554,7 +560,10 @@ static uint32_t grid_for(uint64_t cap) {
 // Pass 0 reads (kin, vin) and writes (kb, vb); later passes ping-pong
 // between (kb, vb) and (kc, vc).  (kc, vc) may alias (kin, vin) when the input
 // may be clobbered.  vin == nullptr means values are the input indices.  When
-// gout is non-null the last pass also writes gout[i] = gsrc[value[i]].
+// gout is non-null the last pass also writes gout[i] = gsrc[value[i]]; with
+// count_ib > 0 the values instead carry a count in bits [count_ib, 32) (all
+// ones: look it up), which the last pass splits off: value[i] & (2^count_ib -
+// 1) goes to the value output, the count (or gsrc[value]) to gout.
 // Returns 1 if the result is in (kb, vb), 2 if in (kc, vc), 0 if key_bits <= 0
 // (nothing done; the input order is already sorted).
 // scratch: radix_scratch_words(cap) uint32.
@@ -563,7 +572,7 @@ uint64_t radix_scratch_words(uint64_t cap) { return 256ull * radix_grid(cap) + 5
 
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
-                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout) {
+                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout, int count_ib) {
   if (key_bits <= 0) return 0;
   const uint32_t G = grid_for<kRadixTile>(cap);
   uint32_t* hist = scratch;
@@ -579,8 +588,8 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
     k_radix_hist<<<G, kSortThreads, 0, s>>>(sk, n_dev, n_host, cap, 8 * p, hist);
     k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot);
     k_radix_scatter<kScatterPairs><<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot,
-                                                              last ? gsrc : nullptr, last ? gout : nullptr, 0,
-                                                              nullptr);
+                                                              last ? gsrc : nullptr, last ? gout : nullptr,
+                                                              last && gout ? count_ib : 0, nullptr);
     sk = dk;
     sv = dv;
     where = (p % 2 == 0) ? 1 : 2;
@@ -598,7 +607,7 @@ int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* 
                             int key_bits, int value_bits, uint32_t* scratch) {
   if (key_bits <= 8 || key_bits > 16 || key_bits - 8 + value_bits > 32)
     return radix_sort_pairs(s, kin, vin, kb, vb, kout, vout, n_dev, n_host, cap, key_bits, scratch, nullptr,
-                            nullptr);
+                            nullptr, 0);
   const uint32_t G = grid_for<kRadixTile>(cap);
   uint32_t* hist = scratch;
   uint32_t* tot_a = scratch + 256 * (size_t)G;

@@ -177,7 +177,8 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
                                                      const ViewDev* __restrict__ views, int v0, int nv, float blur,
                                                      int tw, ProjSplat* __restrict__ proj,
                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ counters) {
+                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ counters,
+                                                     int count_ib) {
   constexpr int NK = (DEG + 1) * (DEG + 1);
   // per wave: the tile-test parameters of its 64 lanes, and the start
   // markers of their bboxes in the wave's joint tile list
@@ -329,7 +330,10 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
     }
     if (live) {
       if (!pass) keys[inst] = 0xffffffffu;
-      vals[inst] = (uint32_t)inst;
+      // with count_ib > 0 the sort value carries the count above the
+      // instance id (all ones when it does not fit: the sort looks it up)
+      const uint32_t cesc = count_ib > 0 ? 0xffffffffu >> count_ib : 0u;
+      vals[inst] = count_ib > 0 ? (uint32_t)inst | ((count < cesc ? count : cesc) << count_ib) : (uint32_t)inst;
       cnt[inst] = count;
     }
     nvis += pass ? 1u : 0u;
@@ -635,15 +639,15 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
 // ------------------------------------------------------------------ launch
 void launch_project_fwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
                         int v0, int nv, float blur, int tw, ProjSplat* proj, uint32_t* keys, uint32_t* vals,
-                        uint32_t* cnt, uint32_t* counters) {
+                        uint32_t* cnt, uint32_t* counters, int count_ib) {
   if (n == 0) return;
   const dim3 grid((n + 255) / 256), block(256);
   switch (deg) {
-    case 0: k_project_fwd<0><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters); break;
-    case 1: k_project_fwd<1><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters); break;
-    case 2: k_project_fwd<2><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters); break;
-    case 3: k_project_fwd<3><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters); break;
-    default: k_project_fwd<4><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters); break;
+    case 0: k_project_fwd<0><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters, count_ib); break;
+    case 1: k_project_fwd<1><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters, count_ib); break;
+    case 2: k_project_fwd<2><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters, count_ib); break;
+    case 3: k_project_fwd<3><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters, count_ib); break;
+    default: k_project_fwd<4><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, tw, proj, keys, vals, cnt, counters, count_ib); break;
   }
 }
 

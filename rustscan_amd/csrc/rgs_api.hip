@@ -20,7 +20,7 @@
 namespace rgs {
 void launch_project_fwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
                         int v0, int nv, float blur, int tw, ProjSplat* proj, uint32_t* keys, uint32_t* vals,
-                        uint32_t* cnt, uint32_t* counters);
+                        uint32_t* cnt, uint32_t* counters, int count_ib);
 void launch_project_bwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
                         int v0, int nv, float blur, float* acc, const uint32_t* keys_by_inst, float* grads,
                         float* stats, int accumulate, const uint32_t* counters);
@@ -28,7 +28,7 @@ uint32_t radix_grid(uint64_t cap);
 uint64_t radix_scratch_words(uint64_t cap);
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
-                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout);
+                     uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout, int count_ib);
 int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                             uint32_t* kout, uint32_t* vout, const uint32_t* n_dev, uint32_t n_host, uint32_t cap,
                             int key_bits, int value_bits, uint32_t* scratch);
@@ -378,21 +378,26 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   const size_t ibase = (size_t)v0 * n;
   const ViewDev* views = w.views.as<ViewDev>();
 
+  // Instance ids are below (v0 + nv) * n; with at most 26 bits the depth
+  // sort's values also carry the instance's tile count in the bits above them.
+  const int inst_bits = (int)ceil_log2((uint64_t)(v0 + nv) * n);
+  const int count_ib = inst_bits <= 26 ? std::max(inst_bits, 1) : 0;
+
   launch_fwd_begin(st, counters);
   launch_fill_u32(st, w.ranges.as<uint32_t>() + 2 * (size_t)tile_first, 0u, 2 * (size_t)tile_count);
   {
     StageTimer t(c, RGS_STAGE_PROJECT_FWD);
     launch_project_fwd(s->deg, st, s->P.as<float>(), s->npad, n, views, v0, nv, blur, vs.tw, w.proj.as<ProjSplat>(),
-                       w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), w.cnt.as<uint32_t>(), counters);
+                       w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), w.cnt.as<uint32_t>(), counters, count_ib);
   }
   // depth sort of all instances of the batch (sorting.rs:34-61; canonical ties: instance order)
   int wd;
   {
     StageTimer t(c, RGS_STAGE_DEPTH_SORT);
-    // the last pass also gathers the tile counts into depth order (scnt)
+    // the last pass also puts the tile counts in depth order (scnt)
     wd = radix_sort_pairs(st, w.keys.as<uint32_t>() + ibase, w.vals.as<uint32_t>() + ibase, w.sk1.as<uint32_t>(),
                           w.sv1.as<uint32_t>(), w.sk2.as<uint32_t>(), w.sv2.as<uint32_t>(), nullptr, nitems, nitems,
-                          32, w.scratch.as<uint32_t>(), w.cnt.as<uint32_t>(), w.scnt.as<uint32_t>());
+                          32, w.scratch.as<uint32_t>(), w.cnt.as<uint32_t>(), w.scnt.as<uint32_t>(), count_ib);
   }
   uint32_t* order = wd == 1 ? w.sv1.as<uint32_t>() : w.sv2.as<uint32_t>();
   // exclusive scan of per-instance tile counts in depth order (prefix_sum.rs:53-136)
@@ -408,8 +413,6 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
                 vs.tw, nitems, (uint32_t)w.cap_isect, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>());
   }
   const int bits = (int)ceil_log2(tile_end);
-  // instance ids are below (v0 + nv) * n
-  const int inst_bits = (int)ceil_log2((uint64_t)(v0 + nv) * n);
   int wt;
   {
     StageTimer t(c, RGS_STAGE_TILE_SORT);
@@ -1151,7 +1154,7 @@ int rgs_sort_pairs_u32(rgs_ctx* c, const uint32_t* keys, const uint32_t* values,
   rc |= ensure(c, scratch, radix_scratch_words((uint64_t)n) * 4);
   if (rc) return RGS_ERR_HIP;
   const int where = radix_sort_pairs(c->stream, keys, values, ok, ov, k2.as<uint32_t>(), v2.as<uint32_t>(), nullptr,
-                                     (uint32_t)n, (uint32_t)n, 32, scratch.as<uint32_t>(), nullptr, nullptr);
+                                     (uint32_t)n, (uint32_t)n, 32, scratch.as<uint32_t>(), nullptr, nullptr, 0);
   if (where == 2) {
     HIPCHK(c, hipMemcpyAsync(ok, k2.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(ov, v2.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
