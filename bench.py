@@ -307,7 +307,7 @@ def main() -> int:
         work = {
             # parameters read once per Gaussian; ProjSplat + key + value + count written per instance
             "project_fwd": ("hbm", (11 + 3 * k) * 4 * n + (64 + 12) * n * nv),
-            "depth_sort": ("hbm", 20 * n * nv * sort_passes_depth + 8 * n * nv),
+            "depth_sort": ("hbm", 20 * n * nv * sort_passes_depth + 4 * n * nv),
             "count_scan": ("hbm", 12 * n * nv),
             # ProjSplat + order + offset read per visible instance; (key, value) written per intersection
             "emit": ("hbm", 72 * V + 8 * M),
