@@ -38,31 +38,47 @@ __global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restri
 
 // engine/optimizer.rs:122-162 for all groups at once: SoA planes, one
 // learning rate per plane (lr * scaling column).  28 bytes of HBM traffic per
-// parameter (read p, g, m, v; write p, m, v).
+// parameter (read p, g, m, v; write p, m, v).  Grid: (column blocks, planes);
+// each thread updates four consecutive columns with 16-byte accesses (planes
+// are npad = multiple of 64 floats long, so every row is 16-byte aligned).
+// Columns past n are written back unchanged.
+__device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float omb1,
+                                         float omb2, float bc1, float bc2, float eps, float lr) {
+  const float gsq = g * g;
+  const float m1 = m * b1 + g * omb1;
+  const float m2 = v * b2 + gsq * omb2;
+  const float mh = m1 / bc1;
+  const float vh = m2 / bc2;
+  const float upd = mh / (rgs_sqrt(vh) + eps);
+  p = p - upd * lr;
+  m = m1;
+  v = m2;
+}
+
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, const float* __restrict__ G,
                                               float* __restrict__ M, float* __restrict__ Vv, uint32_t npad,
-                                              uint32_t n, int nplanes, const float* __restrict__ lr_planes,
-                                              float b1, float b2, float eps, const float* __restrict__ fscal,
+                                              uint32_t n, const float* __restrict__ lr_planes, float b1, float b2,
+                                              float eps, const float* __restrict__ fscal,
                                               const uint32_t* __restrict__ counters) {
   if (counters[kCntStepValid] == 0u) return;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;  // float4 column
+  const uint32_t c0 = 4 * q;
+  if (c0 >= n) return;
   const float omb1 = 1.0f - b1, omb2 = 1.0f - b2;
   const float bc1 = fscal[kFBc1], bc2 = fscal[kFBc2];
-  const size_t total = (size_t)nplanes * npad;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t plane = (uint32_t)(i / npad);
-    const uint32_t col = (uint32_t)(i - (size_t)plane * npad);
-    if (col >= n) continue;
-    const float g = G[i];
-    const float gsq = g * g;
-    const float m1 = M[i] * b1 + g * omb1;
-    const float m2 = Vv[i] * b2 + gsq * omb2;
-    const float mh = m1 / bc1;
-    const float vh = m2 / bc2;
-    const float upd = mh / (rgs_sqrt(vh) + eps);
-    P[i] = P[i] - upd * lr_planes[plane];
-    M[i] = m1;
-    Vv[i] = m2;
-  }
+  const float lr = lr_planes[blockIdx.y];
+  const size_t i4 = ((size_t)blockIdx.y * npad) / 4 + q;
+  float4 p = reinterpret_cast<float4*>(P)[i4];
+  const float4 g = reinterpret_cast<const float4*>(G)[i4];
+  float4 m = reinterpret_cast<float4*>(M)[i4];
+  float4 v = reinterpret_cast<float4*>(Vv)[i4];
+  adam_one(p.x, g.x, m.x, v.x, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
+  if (c0 + 1 < n) adam_one(p.y, g.y, m.y, v.y, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
+  if (c0 + 2 < n) adam_one(p.z, g.z, m.z, v.z, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
+  if (c0 + 3 < n) adam_one(p.w, g.w, m.w, v.w, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
+  reinterpret_cast<float4*>(P)[i4] = p;
+  reinterpret_cast<float4*>(M)[i4] = m;
+  reinterpret_cast<float4*>(Vv)[i4] = v;
 }
 
 // L1 loss and its image gradient for one image (loss.rs:144-196 with the RGB
@@ -123,8 +139,9 @@ void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, ui
                  const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
                  uint32_t* step, int nviews, int check_loss) {
   k_adam_prologue<<<1, 64, 0, s>>>(counters, fscal, step, b1, b2, nviews, check_loss);
-  k_adam<<<blocks_for((size_t)nplanes * npad), 256, 0, s>>>(P, G, M, V, npad, n, nplanes, lr_planes, b1, b2, eps,
-                                                            fscal, counters);
+  if (n == 0 || nplanes <= 0) return;
+  const dim3 grid((npad / 4 + 255) / 256, (uint32_t)nplanes);
+  k_adam<<<grid, 256, 0, s>>>(P, G, M, V, npad, n, lr_planes, b1, b2, eps, fscal, counters);
 }
 
 void launch_l1(hipStream_t s, const float* pred, const float* tgt, uint32_t np, float scale, float* v_out,
