@@ -67,9 +67,13 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* tmp
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ n_dev, uint32_t n_host,
                                                              uint32_t cap, int shift, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[4][256];
-  const int w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 1024; i += kSortThreads) (&h[0][0])[i] = 0;
+  // four copies per wave, picked by lane & 3: keys with few distinct digits
+  // (the high digits of the sorts here) would otherwise pile all 64 lanes'
+  // LDS atomics onto one or two bins
+  constexpr int kCopies = 16;
+  __shared__ uint32_t h[kCopies][256];
+  const int c = ((threadIdx.x >> 6) << 2) | (threadIdx.x & 3);
+  for (int i = threadIdx.x; i < kCopies * 256; i += kSortThreads) (&h[0][0])[i] = 0;
   __syncthreads();
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
@@ -84,15 +88,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
 #pragma unroll 4
   for (uint32_t q = begin / 4 + threadIdx.x; q < vend / 4; q += kSortThreads) {
     const uint4 k = kv[q];
-    atomicAdd(&h[w][(k.x >> shift) & 255u], 1u);
-    atomicAdd(&h[w][(k.y >> shift) & 255u], 1u);
-    atomicAdd(&h[w][(k.z >> shift) & 255u], 1u);
-    atomicAdd(&h[w][(k.w >> shift) & 255u], 1u);
+    atomicAdd(&h[c][(k.x >> shift) & 255u], 1u);
+    atomicAdd(&h[c][(k.y >> shift) & 255u], 1u);
+    atomicAdd(&h[c][(k.z >> shift) & 255u], 1u);
+    atomicAdd(&h[c][(k.w >> shift) & 255u], 1u);
   }
-  for (uint32_t i = vend + threadIdx.x; i < end; i += kSortThreads) atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
+  for (uint32_t i = vend + threadIdx.x; i < end; i += kSortThreads) atomicAdd(&h[c][(keys[i] >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t d = threadIdx.x;
-  hist[(size_t)d * gridDim.x + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+  uint32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < kCopies; ++k) t += h[k][d];
+  hist[(size_t)d * gridDim.x + blockIdx.x] = t;
 }
 
 // grid = 256 blocks (one per digit); G = number of histogram blocks (<= 2048)
@@ -158,12 +165,36 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   uint32_t t0, t1;
   block_tiles<kRadixTile>(n, &t0, &t1);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  // In the tile sort's passes (several tiles per block) the next tile's keys
+  // (and values) are loaded while this tile is ranked, sorted and written, so
+  // their latency overlaps the block's LDS work.  The pairs mode (the depth
+  // sort: about one tile per block) loads each tile when it starts and keeps
+  // the registers (and occupancy) of one tile.
+  constexpr bool kAhead = MODE != kScatterPairs;
+  uint32_t nkey[kRadixRows], nval[kRadixRows];
+  auto load_tile = [&](uint32_t tt) {
+    const uint32_t wb = tt * kRadixTile + (uint32_t)w * (64 * kRadixRows);
+#pragma unroll
+    for (int r = 0; r < kRadixRows; ++r) {
+      const uint32_t idx = wb + (uint32_t)r * 64 + lane;
+      const bool ok = idx < n;
+      nkey[r] = ok ? kin[idx] : 0u;
+      if (MODE != kScatterPackIn) nval[r] = ok ? (vin ? vin[idx] : idx) : 0u;
+    }
+  };
+  if (kAhead && t0 < t1) load_tile(t0);
   for (uint32_t t = t0; t < t1; ++t) {
+    if (!kAhead) load_tile(t);
+    uint32_t key[kRadixRows], val[kRadixRows], loc[kRadixRows];
+#pragma unroll
+    for (int r = 0; r < kRadixRows; ++r) {
+      key[r] = nkey[r];
+      val[r] = MODE != kScatterPackIn ? nval[r] : 0u;
+    }
     for (int i = threadIdx.x; i < 1024; i += kSortThreads) (&wh[0][0])[i] = 0;
     __syncthreads();
     // 1. each wave ranks its 1024 contiguous items row by row (64 per row),
     //    in index order, with a running count per digit
-    uint32_t key[kRadixRows], val[kRadixRows], loc[kRadixRows];
     const uint32_t wbase = t * kRadixTile + (uint32_t)w * (64 * kRadixRows);
     // packed input: low key byte of position wbase (largest d with
     // lo_start[d] <= wbase), then advanced per item
@@ -181,12 +212,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     for (int r = 0; r < kRadixRows; ++r) {
       const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
       const bool ok = idx < n;
-      key[r] = ok ? kin[idx] : 0u;
       if (MODE == kScatterPackIn) {
         while (lo_d < 255u && idx >= lo_next) lo_next = lo_start[++lo_d + 1];
         val[r] = lo_d;
-      } else {
-        val[r] = ok ? (vin ? vin[idx] : idx) : 0u;
       }
       const uint32_t d = (key[r] >> shift) & 255u;
       uint64_t peers = __ballot(ok);
@@ -202,6 +230,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       if (ok && rank == 0) wh[w][d] = old + (uint32_t)__popcll(peers);
       loc[r] = old + rank;
     }
+    if (kAhead && t + 1 < t1) load_tile(t + 1);
     __syncthreads();
     // 2. per digit: local start in the sorted tile (scan over digits), the
     //    base of each wave's run, and the global slot of the digit's run
