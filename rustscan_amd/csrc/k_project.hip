@@ -134,6 +134,7 @@ struct Proj2 {
   float cc[3][3];      // camera covariance [c][r]
   float j20, j21;      // J column 2; J = [[fx*rz, 0], [0, fy*rz], [j20, j21]]
   float jx, jy;        // fx*rz, fy*rz
+  float rz;            // 1 / max(z, 0.01)
   M2 cov2d;            // before blur
 };
 
@@ -152,6 +153,7 @@ __device__ __forceinline__ void project_cov(const Geom& g, const ViewDev& V, flo
     for (int r = 0; r < 3; ++r)
       o.cc[c][r] = (A[0][r] * V.vm[0 * 4 + c] + A[1][r] * V.vm[1 * 4 + c]) + A[2][r] * V.vm[2 * 4 + c];
   const float rz = 1.0f / fmaxf_(mcz, RGS_F(0.01));
+  o.rz = rz;
   const float ux = clampf_(mcx * rz, V.lim_nx, V.lim_px);
   const float uy = clampf_(mcy * rz, V.lim_ny, V.lim_py);
   o.jx = V.fx * rz;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
       const float opacity = sig * comp;
       pass = !(opacity < RGS_F(1.0 / 255.0));
       if (pass) {
-        const float rzm = 1.0f / mcz;
+        const float rzm = pc.rz;  // = 1 / mcz: mcz >= 0.01 here
         m2x = V.fx * mcx * rzm + V.cx;
         m2y = V.fy * mcy * rzm + V.cy;
         const M2 con = inverse2x2(c2);

@@ -41,9 +41,10 @@ __device__ __forceinline__ M2 inverse2x2(const M2& m) {
 // helpers.wgsl:204-216 get_tile_bbox: (min_x, min_y, max_x, max_y) in tiles
 __device__ __forceinline__ void tile_bbox(float cx_, float cy_, float ex, float ey, uint32_t tbx, uint32_t tby,
                                           int tw, uint32_t bb[4]) {
-  const float twf = (float)tw;
-  const float c0 = cx_ / twf, c1 = cy_ / twf;
-  const float e0 = ex / twf, e1 = ey / twf;
+  // tw is a power of two, so x * (1/tw) rounds exactly as x / tw does
+  const float inv = 1.0f / (float)tw;
+  const float c0 = cx_ * inv, c1 = cy_ * inv;
+  const float e0 = ex * inv, e1 = ey * inv;
   const float bx = (float)tbx, by = (float)tby;
   bb[0] = (uint32_t)clampf_(c0 - e0, 0.0f, bx);
   bb[1] = (uint32_t)clampf_(c1 - e1, 0.0f, by);
