@@ -1079,7 +1079,9 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
       float ms = 0.0f;
       hipEventElapsedTime(&ms, e0, e1);
       ms /= reps;
-      if (ms < best_ms) {
+      // a later candidate must win by 3% (configurations within timing
+      // noise keep the earlier one: smaller tile, then smaller batch)
+      if (ms < best_ms * 0.97f) {
         best_ms = ms;
         best = rc_cfg;
       }
