@@ -208,7 +208,12 @@ enum {
 int rgs_debug_get(rgs_ctx* ctx, int what, int view, void* dst, int64_t* count);
 /* flags bit 0: keep a copy of the rasterizer accumulators of each backward
  * (RGS_DEBUG_RASTER_GRADS); costs one extra copy per backward.
- * flags bit 1: count compositing fragments in eager launches. */
+ * flags bit 1: count compositing fragments in eager launches.
+ * flags bit 2: plain sorts: the depth sort gathers the tile counts and the
+ * tile sort moves (key, value) pairs in every pass, instead of carrying the
+ * counts in the sort values and packing the middle pass (same results).
+ * flags bit 3: no block culling in the compositing kernels: every wave
+ * walks every splat of its tile (same results). */
 int rgs_set_debug(rgs_ctx* ctx, int flags);
 /* Fragment counts since the last reset (bit 1 of rgs_set_debug), 8 entries:
  * [0] forward (pixel, splat) pairs evaluated, [1] forward pairs blended,

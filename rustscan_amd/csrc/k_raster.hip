@@ -299,7 +299,8 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
           s2[j] = make_float2(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin);
           sg[j] = inst - (uint32_t)v * A.n;
           sv[j] = 0u;
-          bits = block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy);
+          bits = A.cull ? block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy)
+                        : (1u << ((RW / 8) * (RW / 8))) - 1u;
         }
         publish_block_masks<RW, BATCH>(swords, j0, bits);
       }
@@ -454,7 +455,8 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
           s1[j] = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
           s2[j] = make_float2(clamp_colour(q.p2.x), q.p2.y + kSkipMargin);
           si[j] = inst;
-          bits = block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy);
+          bits = A.cull ? block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy)
+                        : (1u << ((RW / 8) * (RW / 8))) - 1u;
         }
         publish_block_masks<RW, BATCH>(swords, j0, bits);
       }

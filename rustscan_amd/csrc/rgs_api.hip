@@ -159,6 +159,8 @@ struct rgs_ctx {
   // fragment counters (device, kFragCounters x u64) when enabled
   DevBuf frag;
   bool count_frags = false;
+  bool plain_sorts = false;  // debug: unpacked tile sort, count gather in the depth sort
+  bool no_cull = false;      // debug: no per-block culling in the compositing kernels
 };
 
 namespace {
@@ -381,7 +383,7 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   // Instance ids are below (v0 + nv) * n; with at most 26 bits the depth
   // sort's values also carry the instance's tile count in the bits above them.
   const int inst_bits = (int)ceil_log2((uint64_t)(v0 + nv) * n);
-  const int count_ib = inst_bits <= 26 ? std::max(inst_bits, 1) : 0;
+  const int count_ib = (inst_bits <= 26 && !c->plain_sorts) ? std::max(inst_bits, 1) : 0;
 
   launch_fwd_begin(st, counters);
   launch_fill_u32(st, w.ranges.as<uint32_t>() + 2 * (size_t)tile_first, 0u, 2 * (size_t)tile_count);
@@ -416,9 +418,15 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   int wt;
   {
     StageTimer t(c, RGS_STAGE_TILE_SORT);
-    wt = radix_sort_pairs_packed(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(),
-                                 w.iv1.as<uint32_t>(), w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect,
-                                 0, (uint32_t)w.cap_isect, bits, inst_bits, w.scratch.as<uint32_t>());
+    if (c->plain_sorts)
+      wt = radix_sort_pairs(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(), w.iv1.as<uint32_t>(),
+                            w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), counters + kCntIsect, 0,
+                            (uint32_t)w.cap_isect, bits, w.scratch.as<uint32_t>(), nullptr, nullptr, 0);
+    else
+      wt = radix_sort_pairs_packed(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(),
+                                   w.iv1.as<uint32_t>(), w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(),
+                                   counters + kCntIsect, 0, (uint32_t)w.cap_isect, bits, inst_bits,
+                                   w.scratch.as<uint32_t>());
   }
   uint32_t* skeys = wt == 1 ? w.ik1.as<uint32_t>() : w.ik0.as<uint32_t>();
   uint32_t* svals = wt == 1 ? w.iv1.as<uint32_t>() : w.iv0.as<uint32_t>();
@@ -437,6 +445,7 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   A.tile_first = tile_first;
   A.tile_count = tile_count;
   A.queue = c->raster.persistent ? counters + kCntQueueFwd : nullptr;
+  A.cull = c->no_cull ? 0 : 1;
   A.n = n;
   A.visible = s->visible.as<float>();
   A.loss = with_loss ? w.fscal.as<float>() + kFLossBase : nullptr;
@@ -474,6 +483,7 @@ int enqueue_backward(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, int v0, int n
   A.tile_first = tile_first;
   A.tile_count = tile_count;
   A.queue = c->raster.persistent ? counters + kCntQueueBwd : nullptr;
+  A.cull = c->no_cull ? 0 : 1;
   A.n = n;
   A.visible = nullptr;
   A.loss = nullptr;
@@ -1319,6 +1329,8 @@ int rgs_set_debug(rgs_ctx* c, int flags) {
   if (!c) return RGS_ERR_INVALID;
   c->keep_raster_grads = (flags & 1) != 0;
   c->count_frags = (flags & 2) != 0;
+  c->plain_sorts = (flags & 4) != 0;
+  c->no_cull = (flags & 8) != 0;
   if (c->count_frags && !c->frag.p) {
     if (ensure(c, c->frag, kFragCounters * sizeof(unsigned long long))) return RGS_ERR_HIP;
     HIPCHK(c, hipMemsetAsync(c->frag.p, 0, kFragCounters * sizeof(unsigned long long), c->stream));

@@ -80,6 +80,7 @@ struct RasterArgs {
   int v0, nv;
   uint32_t tile_first, tile_count;
   uint32_t* queue;  // persistent work counter, or null for one workgroup per tile
+  int cull;         // nonzero: skip splats per 8x8 block that provably miss it
   uint32_t n;       // Gaussians per view; instance = view * n + gid
   float* visible;   // [n] OR over views, or null
   float* loss;      // per-view sums of |pred - target|, or null

@@ -121,8 +121,12 @@ class Context:
         self.check(self.lib.rgs_get_timing(self.h, ms, calls, 1 if reset else 0))
         return {name: (ms[i], calls[i]) for i, name in enumerate(L.STAGES)}
 
-    def set_debug(self, keep_raster_grads: bool = False, count_fragments: bool = False):
-        flags = (1 if keep_raster_grads else 0) | (2 if count_fragments else 0)
+    def set_debug(self, keep_raster_grads: bool = False, count_fragments: bool = False, plain_sorts: bool = False,
+                  no_cull: bool = False):
+        """Debug switches (rgs_set_debug): keep the rasterizer accumulators,
+        count fragments, plain (unpacked) sorts, no block culling."""
+        flags = (1 if keep_raster_grads else 0) | (2 if count_fragments else 0) | (4 if plain_sorts else 0) | \
+            (8 if no_cull else 0)
         self.check(self.lib.rgs_set_debug(self.h, flags))
 
     def fragment_counts(self, reset: bool = True) -> dict:

@@ -192,3 +192,49 @@ def test_forward_needle_splats(hip_ctx, orc, tile, batch):
     sc, cam = needle_scene()
     say(f"forward needle splats, tile {tile} batch {batch}")
     check_forward(hip_ctx, orc, sc, cam, tile=tile, batch=batch)
+
+
+# ---- the fast paths against their plain forms: the packed tile sort and the
+# count-carrying depth sort against plain (key, value) sorts, and the per-block
+# culling against every wave walking every splat.  Results must be identical.
+
+
+@pytest.mark.parametrize("tile,batch", [(16, 256), (32, 128)])
+def test_packed_sorts_and_culling_match_plain(hip_ctx, tile, batch):
+    import torch
+
+    from rustscan_amd import _lib as L
+    from rustscan_amd.rasterizer import DeviceSplats, l1_loss_grad, render_backward, render_forward
+
+    sc, cams = ring_scene(n=32768, seed=33)
+    target = torch.rand((cams[0].height, cams[0].width, 3), device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(34))
+    hip_ctx.set_raster(tile, batch, True)
+    what = [L.RGS_DEBUG_DEPTH_ORDER, L.RGS_DEBUG_TILE_COUNTS, L.RGS_DEBUG_ISECT_TILES, L.RGS_DEBUG_ISECT_GIDS,
+            L.RGS_DEBUG_TILE_RANGES]
+
+    def run(plain):
+        hip_ctx.set_debug(plain_sorts=plain, no_cull=plain)
+        try:
+            ds = DeviceSplats(hip_ctx, sc)
+            imgs = render_forward(hip_ctx, ds, cams)
+            dbg = [[hip_ctx.debug(w, v) for w in what] for v in range(len(cams))]
+            _, v_out = l1_loss_grad(hip_ctx, imgs[0], target)
+            grads = render_backward(hip_ctx, ds, [v_out] + [torch.zeros_like(im) for im in imgs[1:]])
+            out = ([im.cpu().numpy() for im in imgs], dbg, grads, ds.visible())
+            ds.close()
+            return out
+        finally:
+            hip_ctx.set_debug()
+
+    say(f"packed sorts and culling vs plain, tile {tile} batch {batch}")
+    fast, plain = run(False), run(True)
+    for a, b in zip(fast[0], plain[0]):
+        np.testing.assert_array_equal(a, b)
+    for va, vb in zip(fast[1], plain[1]):
+        for a, b in zip(va, vb):
+            np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(fast[3], plain[3])
+    errs = [rel_l2(a, b) for a, b in zip(fast[2], plain[2])]
+    say("  grads rel-L2 " + " ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) <= 1e-6
