@@ -7,7 +7,8 @@
 //   k_radix_hist    per-block digit histograms          hist[digit][block]
 //   k_radix_spine   one block per digit: exclusive scan of its row over blocks
 //   k_radix_scatter per 4096-item tile: each wave ranks its 1024 contiguous
-//                   items with ballots (bitwise match of the digit), the tile
+//                   items (equal digits found by ballots or by peer masks
+//                   per digit in LDS), the tile
 //                   is sorted by digit in LDS, and written out so that
 //                   consecutive threads store consecutive slots of a bucket.
 // Every kernel reads the item count from device memory (clamped to capacity),
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   __shared__ uint32_t base[256];     // next global slot of each digit for this block
   __shared__ uint32_t gbase[256];    // global slot of local position 0 of each digit's run in the tile
   __shared__ uint32_t wh[4][256];    // per-wave digit counts, then per-wave local bases
-  __shared__ uint32_t sk[kRadixTile], sv[kRadixTile];
+  __shared__ __align__(16) uint32_t sk[kRadixTile];  // 8-byte peer masks during the ranking
+  __shared__ uint32_t sv[kRadixTile];
   __shared__ uint32_t lo_start[MODE == kScatterPackIn ? 257 : 1];  // first position of each low byte
   __shared__ uint32_t tmp[8];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -171,6 +173,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   // sort: about one tile per block) loads each tile when it starts and keeps
   // the registers (and occupancy) of one tile.
   constexpr bool kAhead = MODE != kScatterPairs;
+  // Ranking: peer masks in LDS for the tile sort's passes (several tiles per
+  // block, loads in flight), ballots for the depth sort (latency-bound: one
+  // tile per block).
+  constexpr bool kMaskRank = MODE != kScatterPairs;
   uint32_t nkey[kRadixRows], nval[kRadixRows];
   auto load_tile = [&](uint32_t tt) {
     const uint32_t wb = tt * kRadixTile + (uint32_t)w * (64 * kRadixRows);
@@ -191,7 +197,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       key[r] = nkey[r];
       val[r] = MODE != kScatterPackIn ? nval[r] : 0u;
     }
-    for (int i = threadIdx.x; i < 1024; i += kSortThreads) (&wh[0][0])[i] = 0;
+    // per-wave peer masks of the ranking (in sk, unused until step 3)
+    uint64_t* pm = reinterpret_cast<uint64_t*>(sk) + (size_t)w * 256;
+    for (int i = threadIdx.x; i < 1024; i += kSortThreads) {
+      (&wh[0][0])[i] = 0;
+      if (kMaskRank) reinterpret_cast<uint64_t*>(sk)[i] = 0ull;
+    }
     __syncthreads();
     // 1. each wave ranks its 1024 contiguous items row by row (64 per row),
     //    in index order, with a running count per digit
@@ -217,17 +228,33 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         val[r] = lo_d;
       }
       const uint32_t d = (key[r] >> shift) & 255u;
-      uint64_t peers = __ballot(ok);
+      uint64_t peers;
+      if (kMaskRank) {
+        // lanes of the same digit: each ORs its bit into the digit's mask
+        // (the result does not depend on the order) and reads it back; the
+        // first of them clears it after the count update below
+        if (ok) atomicOr(reinterpret_cast<unsigned long long*>(&pm[d]), 1ull << lane);
+        wave_lds_fence();
+        peers = ok ? pm[d] : 0ull;
+      } else {
+        // lanes of the same digit by a bitwise ballot match
+        peers = __ballot(ok);
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        for (int b = 0; b < 8; ++b) {
+          const bool bit = (d >> b) & 1u;
+          const uint64_t m = __ballot(bit);
+          peers &= bit ? m : ~m;
+        }
       }
       const uint32_t rank = (uint32_t)__popcll(peers & lt);
       uint32_t old = 0;
       if (ok) old = wh[w][d];
-      if (ok && rank == 0) wh[w][d] = old + (uint32_t)__popcll(peers);
+      if (kMaskRank) wave_lds_fence();
+      if (ok && rank == 0) {
+        wh[w][d] = old + (uint32_t)__popcll(peers);
+        if (kMaskRank) pm[d] = 0ull;
+      }
+      if (kMaskRank) wave_lds_fence();
       loc[r] = old + rank;
     }
     if (kAhead && t + 1 < t1) load_tile(t + 1);
