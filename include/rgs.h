@@ -178,7 +178,9 @@ int rgs_autotune(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, const floa
 /* Counts of the last forward per view (n_views entries); synchronous. */
 int rgs_get_counts(rgs_ctx* ctx, rgs_counts* out, int n_views);
 /* Number of steps whose update was skipped (capacity overflow or non-finite
- * loss) since the context was created; synchronous. */
+ * loss) since the context was created; synchronous.  A synchronous
+ * rgs_train_step (loss_out non-null) that overflows grows its buffers and
+ * redoes the step, which is then not counted. */
 int64_t rgs_skipped_steps(rgs_ctx* ctx);
 
 /* ---- GPU primitives (gpu_primitives/, device pointers) ---------------------

@@ -1030,6 +1030,9 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
     if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, (size_t)c->h_counters[kCntIsectMax] * 5 / 4 + 1024)))
       return rc;
     if (attempt == 2) return fail(c, RGS_ERR_CAPACITY, "intersection buffer could not be sized");
+    // the overflowed attempt counted itself as skipped; it is redone now
+    if (c->h_counters[kCntSkipped] > 0)
+      launch_set_u32(c->stream, c->ws.counters.as<uint32_t>() + kCntSkipped, c->h_counters[kCntSkipped] - 1u);
   }
   c->fwd_valid = true;
   c->fwd_splats = s;
