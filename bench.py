@@ -181,8 +181,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
-              "view-parallel data parallelism, described in DESIGN.md, and is not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (more GPUs would split the "
+              "views of a step across ranks and reduce the Gaussian gradients; that path is designed, not built)",
+              file=sys.stderr)
         return 2
 
     import torch
