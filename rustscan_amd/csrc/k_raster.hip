@@ -217,6 +217,33 @@ __device__ __forceinline__ void publish_block_masks(uint64_t (*words)[BATCH / 64
   }
 }
 
+// Position of this thread's item among the flagged items of the workgroup
+// (in thread order); *total = their number.  Every thread must call it.
+// REUSE: s_wc may be written again before every thread has passed another
+// barrier, so the function ends with one.
+template <int NT, bool REUSE>
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* s_wc, uint32_t* total) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(flag);
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t below = (uint32_t)__popcll(m & (lane ? (~0ull >> (64u - lane)) : 0ull));
+  if (NT == 64) {
+    *total = (uint32_t)__popcll(m);
+    return below;
+  }
+  if (lane == 0) s_wc[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    const uint32_t c = s_wc[i];
+    off += (uint32_t)i < w ? c : 0u;
+    tot += c;
+  }
+  if (REUSE) __syncthreads();
+  *total = tot;
+  return off + below;
+}
+
 // Calls body(s0[j], s1[j], s2[j], j) for the staged splats j < cnt whose bit
 // is set in this wave's words, in increasing j, while alive() holds for some
 // lane.  Two splats per trip, the next record fetched one splat ahead into
@@ -382,7 +409,9 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
   __shared__ float4 s1[BATCH];  // c/2, opacity, r, g
   __shared__ float2 s2[BATCH];  // b, exp-skip bound
   __shared__ uint32_t si[BATCH];
-  __shared__ float sacc[BATCH * SA];
+  __shared__ float sacc[BATCH * SA];  // all zero between batches
+  __shared__ uint32_t sfl[BATCH];     // staged splats with contributions
+  __shared__ uint32_t s_wc[NT / 64];
   __shared__ uint64_t swords[(RW / 8) * (RW / 8)][BATCH / 64];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
@@ -394,6 +423,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
 #pragma unroll
   for (int a = 0; a < 3; ++a)
     if (!REFINE && slot[a] == 9) slot[a] = 12;  // coverage count rides in the refine slot
+  for (uint32_t k = threadIdx.x; k < (uint32_t)(BATCH * SA); k += NT) sacc[k] = 0.0f;
   uint32_t iter = 0, item;
   while (next_item(A, A.tile_count * (SUB * SUB), &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item / (SUB * SUB), sub = item % (SUB * SUB);
@@ -460,7 +490,6 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         }
         publish_block_masks<RW, BATCH>(swords, j0, bits);
       }
-      for (uint32_t k = tid; k < cnt * SA; k += NT) sacc[k] = 0.0f;
       __syncthreads();
       // Same walk as the forward: the splats that can reach the wave's block.
       auto accumulate = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
@@ -542,13 +571,27 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
       };
       for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, accumulate);
       __syncthreads();
-      // Flush: 16 consecutive threads per splat, one float each, so each
-      // atomic instruction covers whole 64-byte accumulator lines.
-      for (uint32_t e = tid; e < cnt * SA; e += NT) {
-        const uint32_t j = e / SA, k = e % SA;
+      // Flush the splats that received contributions (coverage slot 12 is
+      // nonzero exactly then): 16 consecutive threads per splat, one float
+      // each, so each atomic instruction covers whole 64-byte accumulator
+      // lines.  Each slot is cleared as it is read.
+      uint32_t nf = 0;
+      for (uint32_t j0 = 0; j0 < (uint32_t)BATCH; j0 += NT) {
+        const uint32_t j = j0 + tid;
+        const bool has = j < cnt && sacc[j * SA + 12] != 0.0f;
+        uint32_t tot;
+        // one call per batch when BATCH <= NT: the next write of s_wc comes
+        // after the barriers of the next batch
+        const uint32_t r = nf + block_rank<NT, (BATCH > NT)>(has, s_wc, &tot);
+        if (has) sfl[r] = j;
+        nf += tot;
+      }
+      __syncthreads();
+      for (uint32_t e = tid; e < nf * SA; e += NT) {
+        const uint32_t j = sfl[e / SA], k = e % SA;
+        float x = sacc[j * SA + k];
+        sacc[j * SA + k] = 0.0f;
         if (k >= (uint32_t)NA) continue;
-        if (sacc[j * SA + 12] == 0.0f) continue;
-        float x = sacc[e];
         if (k == 2 || k == 4) x *= 0.5f;
         if (k >= 5 && k <= 7) {
           const float col = k == 5 ? s1[j].z : (k == 6 ? s1[j].w : s2[j].x);

@@ -1084,7 +1084,7 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
       int r = rgs_train_step(c, s, cams, targets, nv, cfg, nullptr);  // warm-up (sizes, captures)
       if (r) continue;
       hipEventRecord(e0, c->stream);
-      const int reps = 3;
+      const int reps = 5;
       for (int k = 0; k < reps && !r; ++k) r = rgs_train_step(c, s, cams, targets, nv, cfg, nullptr);
       hipEventRecord(e1, c->stream);
       hipEventSynchronize(e1);
@@ -1092,9 +1092,9 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
       float ms = 0.0f;
       hipEventElapsedTime(&ms, e0, e1);
       ms /= reps;
-      // a later candidate must win by 3% (configurations within timing
+      // a later candidate must win by 1% (configurations within timing
       // noise keep the earlier one: smaller tile, then smaller batch)
-      if (ms < best_ms * 0.97f) {
+      if (ms < best_ms * 0.99f) {
         best_ms = ms;
         best = rc_cfg;
       }
