@@ -23,8 +23,13 @@ namespace rgs {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortTile = 1024;  // scan: items per block iteration (256 threads x 4)
-constexpr int kRadixRows = 16;   // radix: items per lane per tile
+constexpr int kRadixRows = 16;   // radix: items per lane per tile (pairs sorts)
 constexpr int kRadixTile = kSortThreads * kRadixRows;  // 4096 items; each wave owns 1024 contiguous
+// The packed tile-sort passes use 2048-item tiles: half the registers per
+// lane, so more waves hide their memory latency (measured faster; the
+// latency-bound depth sort is faster with 4096).
+constexpr int kRadixRowsPacked = 8;
+constexpr int kRadixTilePacked = kSortThreads * kRadixRowsPacked;
 
 __device__ __forceinline__ uint32_t dev_count(const uint32_t* n_dev, uint32_t n_host, uint32_t cap) {
   const uint32_t n = n_dev ? *n_dev : n_host;
@@ -65,6 +70,7 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* tmp
   return wbase + x - v;
 }
 
+template <int TILE>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ n_dev, uint32_t n_host,
                                                              uint32_t cap, int shift, uint32_t* __restrict__ hist) {
@@ -78,9 +84,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
   __syncthreads();
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
-  block_tiles<kRadixTile>(n, &t0, &t1);
-  const uint32_t begin = t0 * kRadixTile;
-  const uint32_t end = t1 * kRadixTile < n ? t1 * kRadixTile : n;
+  block_tiles<TILE>(n, &t0, &t1);
+  const uint32_t begin = t0 * TILE;
+  const uint32_t end = t1 * TILE < n ? t1 * TILE : n;
   // four keys per thread per step (vector loads), several steps in flight;
   // scalar loads when the range does not start 16-byte aligned
   const bool aligned = ((reinterpret_cast<uintptr_t>(keys) + 4ull * begin) & 15u) == 0;
@@ -147,8 +153,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   __shared__ uint32_t base[256];     // next global slot of each digit for this block
   __shared__ uint32_t gbase[256];    // global slot of local position 0 of each digit's run in the tile
   __shared__ uint32_t wh[4][256];    // per-wave digit counts, then per-wave local bases
-  __shared__ __align__(16) uint32_t sk[kRadixTile];  // 8-byte peer masks during the ranking
-  __shared__ uint32_t sv[kRadixTile];
+  constexpr int ROWS = MODE == kScatterPairs ? kRadixRows : kRadixRowsPacked;
+  constexpr int TILE = kSortThreads * ROWS;
+  __shared__ __align__(16) uint32_t sk[TILE];  // 8-byte peer masks during the ranking
+  __shared__ uint32_t sv[TILE];
   __shared__ uint32_t lo_start[MODE == kScatterPackIn ? 257 : 1];  // first position of each low byte
   __shared__ uint32_t tmp[8];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   const uint32_t vmask = (ib <= 0 || ib >= 32) ? 0xffffffffu : ((1u << ib) - 1u);
   const uint32_t n = dev_count(n_dev, n_host, cap);
   uint32_t t0, t1;
-  block_tiles<kRadixTile>(n, &t0, &t1);
+  block_tiles<TILE>(n, &t0, &t1);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   // In the tile sort's passes (several tiles per block) the next tile's keys
   // (and values) are loaded while this tile is ranked, sorted and written, so
@@ -177,11 +185,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   // block, loads in flight), ballots for the depth sort (latency-bound: one
   // tile per block).
   constexpr bool kMaskRank = MODE != kScatterPairs;
-  uint32_t nkey[kRadixRows], nval[kRadixRows];
+  uint32_t nkey[ROWS], nval[ROWS];
   auto load_tile = [&](uint32_t tt) {
-    const uint32_t wb = tt * kRadixTile + (uint32_t)w * (64 * kRadixRows);
+    const uint32_t wb = tt * TILE + (uint32_t)w * (64 * ROWS);
 #pragma unroll
-    for (int r = 0; r < kRadixRows; ++r) {
+    for (int r = 0; r < ROWS; ++r) {
       const uint32_t idx = wb + (uint32_t)r * 64 + lane;
       const bool ok = idx < n;
       nkey[r] = ok ? kin[idx] : 0u;
@@ -191,9 +199,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   if (kAhead && t0 < t1) load_tile(t0);
   for (uint32_t t = t0; t < t1; ++t) {
     if (!kAhead) load_tile(t);
-    uint32_t key[kRadixRows], val[kRadixRows], loc[kRadixRows];
+    uint32_t key[ROWS], val[ROWS], loc[ROWS];
 #pragma unroll
-    for (int r = 0; r < kRadixRows; ++r) {
+    for (int r = 0; r < ROWS; ++r) {
       key[r] = nkey[r];
       val[r] = MODE != kScatterPackIn ? nval[r] : 0u;
     }
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __syncthreads();
     // 1. each wave ranks its 1024 contiguous items row by row (64 per row),
     //    in index order, with a running count per digit
-    const uint32_t wbase = t * kRadixTile + (uint32_t)w * (64 * kRadixRows);
+    const uint32_t wbase = t * TILE + (uint32_t)w * (64 * ROWS);
     // packed input: low key byte of position wbase (largest d with
     // lo_start[d] <= wbase), then advanced per item
     uint32_t lo_d = 0, lo_next = 0;
@@ -220,7 +228,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       lo_next = lo_start[a + 1];
     }
 #pragma unroll
-    for (int r = 0; r < kRadixRows; ++r) {
+    for (int r = 0; r < ROWS; ++r) {
       const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
       const bool ok = idx < n;
       if (MODE == kScatterPackIn) {
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __syncthreads();
     // 3. sort the tile by digit in LDS (stable: wave order, then rank)
 #pragma unroll
-    for (int r = 0; r < kRadixRows; ++r) {
+    for (int r = 0; r < ROWS; ++r) {
       const uint32_t idx = wbase + (uint32_t)r * 64 + lane;
       if (idx < n) {
         const uint32_t p = wh[w][(key[r] >> shift) & 255u] + loc[r];
@@ -288,7 +296,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __syncthreads();
     // 4. write out: thread i stores local position i, so each digit's run
     //    goes to consecutive global slots from consecutive threads
-    const uint32_t tn = min((uint32_t)kRadixTile, n - t * kRadixTile);
+    const uint32_t tn = min((uint32_t)TILE, n - t * TILE);
 #pragma unroll 4
     for (uint32_t i = threadIdx.x; i < tn; i += kSortThreads) {
       const uint32_t k = sk[i];
@@ -623,7 +631,8 @@ static uint32_t grid_for(uint64_t cap) {
 // Returns 1 if the result is in (kb, vb), 2 if in (kc, vc), 0 if key_bits <= 0
 // (nothing done; the input order is already sorted).
 // scratch: radix_scratch_words(cap) uint32.
-uint32_t radix_grid(uint64_t cap) { return grid_for<kRadixTile>(cap); }
+// (the larger of the two tile sizes' grids: scratch for either sort)
+uint32_t radix_grid(uint64_t cap) { return grid_for<kRadixTilePacked>(cap); }
 uint64_t radix_scratch_words(uint64_t cap) { return 256ull * radix_grid(cap) + 512; }
 
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
@@ -641,7 +650,7 @@ int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, ui
     uint32_t* dk = (p % 2 == 0) ? kb : kc;
     uint32_t* dv = (p % 2 == 0) ? vb : vc;
     const bool last = p + 1 == passes;
-    k_radix_hist<<<G, kSortThreads, 0, s>>>(sk, n_dev, n_host, cap, 8 * p, hist);
+    k_radix_hist<kRadixTile><<<G, kSortThreads, 0, s>>>(sk, n_dev, n_host, cap, 8 * p, hist);
     k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot);
     k_radix_scatter<kScatterPairs><<<G, kSortThreads, 0, s>>>(sk, sv, dk, dv, n_dev, n_host, cap, 8 * p, hist, tot,
                                                               last ? gsrc : nullptr, last ? gout : nullptr,
@@ -664,16 +673,16 @@ int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* 
   if (key_bits <= 8 || key_bits > 16 || key_bits - 8 + value_bits > 32)
     return radix_sort_pairs(s, kin, vin, kb, vb, kout, vout, n_dev, n_host, cap, key_bits, scratch, nullptr,
                             nullptr, 0);
-  const uint32_t G = grid_for<kRadixTile>(cap);
+  const uint32_t G = grid_for<kRadixTilePacked>(cap);
   uint32_t* hist = scratch;
   uint32_t* tot_a = scratch + 256 * (size_t)G;
   uint32_t* tot_b = tot_a + 256;
   uint32_t* mid = kb;
-  k_radix_hist<<<G, kSortThreads, 0, s>>>(kin, n_dev, n_host, cap, 0, hist);
+  k_radix_hist<kRadixTilePacked><<<G, kSortThreads, 0, s>>>(kin, n_dev, n_host, cap, 0, hist);
   k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot_a);
   k_radix_scatter<kScatterPackOut><<<G, kSortThreads, 0, s>>>(kin, vin, mid, nullptr, n_dev, n_host, cap, 0, hist,
                                                               tot_a, nullptr, nullptr, value_bits, nullptr);
-  k_radix_hist<<<G, kSortThreads, 0, s>>>(mid, n_dev, n_host, cap, value_bits, hist);
+  k_radix_hist<kRadixTilePacked><<<G, kSortThreads, 0, s>>>(mid, n_dev, n_host, cap, value_bits, hist);
   k_radix_spine<<<256, kSortThreads, 0, s>>>(hist, G, tot_b);
   k_radix_scatter<kScatterPackIn><<<G, kSortThreads, 0, s>>>(mid, nullptr, kout, vout, n_dev, n_host, cap,
                                                              value_bits, hist, tot_b, nullptr, nullptr, value_bits,
