@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_down(const uint32_t* __re
 //    (long, contiguous) runs go straight to memory.
 // Writes past `cap` are dropped and flagged; the step is then marked invalid
 // so no parameter is updated.
-constexpr uint32_t kEmitStage = 4096;
+constexpr uint32_t kEmitStage = 2048;  // staged output slots per window (8 KB + 8 KB of LDS)
 constexpr uint32_t kEmptySlot = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj, const uint32_t* __restrict__ order,
