@@ -402,7 +402,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
   // accumulator: 0-1 v_xy, 2-4 v_conic, 5-7 v_rgb, 8 opacity term, 9 refine,
   // 10-11 |v_xy|, 12 coverage count.
   constexpr int NA = 13;
-  constexpr int SA = 16;  // LDS stride per splat
+  constexpr int SA = NA;  // LDS stride per splat (13 floats: less LDS per workgroup, more resident waves)
   __shared__ float4 s0[BATCH];  // x, y, a/2, b
   // colours as stored by clamp_colour(): max(c, 0), with the sign bit set
   // where c < 0 (the colour clamp's gradient gate)
@@ -572,9 +572,9 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
       for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, accumulate);
       __syncthreads();
       // Flush the splats that received contributions (coverage slot 12 is
-      // nonzero exactly then): 16 consecutive threads per splat, one float
-      // each, so each atomic instruction covers whole 64-byte accumulator
-      // lines.  Each slot is cleared as it is read.
+      // nonzero exactly then): 13 consecutive threads per splat, one float
+      // each, so the atomics of a splat go to one 64-byte accumulator line.
+      // Each slot is cleared as it is read.
       uint32_t nf = 0;
       for (uint32_t j0 = 0; j0 < (uint32_t)BATCH; j0 += NT) {
         const uint32_t j = j0 + tid;
@@ -591,7 +591,6 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         const uint32_t j = sfl[e / SA], k = e % SA;
         float x = sacc[j * SA + k];
         sacc[j * SA + k] = 0.0f;
-        if (k >= (uint32_t)NA) continue;
         if (k == 2 || k == 4) x *= 0.5f;
         if (k >= 5 && k <= 7) {
           const float col = k == 5 ? s1[j].z : (k == 6 ? s1[j].w : s2[j].x);
