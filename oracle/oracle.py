@@ -68,6 +68,8 @@ def lib():
         L.orc_get_isect.argtypes = [c_void_p, c_void_p, c_void_p]
         L.orc_backward.restype = c_int
         L.orc_backward.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
+        L.orc_backward_f64.restype = c_int
+        L.orc_backward_f64.argtypes = [c_void_p, c_void_p, c_int, c_void_p] + [c_void_p] * 4
         L.orc_l1_loss_grad.restype = c_double
         L.orc_l1_loss_grad.argtypes = [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p]
         L.orc_adam_group.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
@@ -199,6 +201,24 @@ class Render:
         if rc:
             raise ValueError("oracle backward failed")
         return dict(v_transforms=vt, v_sh=vs, v_raw_opac=vo, screen_stats=st, v_splats=vsp, screen_splats=ssp)
+
+    def backward_f64(self, v_out: np.ndarray, mode: int = 0, v_splats=None):
+        """The backward replayed in double over this FP32 forward (same lists,
+        splats and FP32 skip/stop decisions).  With v_splats ([V,10] FP32
+        compact splat gradients) only the projection backward runs in double,
+        on those values."""
+        vt = np.zeros((self.n, 10), np.float64)
+        vs = np.zeros((self.n, self.k, 3), np.float64)
+        vo = np.zeros(self.n, np.float64)
+        V = self.stats["num_visible"]
+        vsp = np.zeros((V, 10), np.float64)
+        v_out = _f32(v_out)
+        vin = _f32(v_splats) if v_splats is not None else None
+        rc = lib().orc_backward_f64(self.h, v_out.ctypes.data, mode, vin.ctypes.data if vin is not None else None,
+                                    vt.ctypes.data, vs.ctypes.data, vo.ctypes.data, vsp.ctypes.data)
+        if rc:
+            raise ValueError("oracle backward failed")
+        return dict(v_transforms=vt, v_sh=vs, v_raw_opac=vo, v_splats=vsp)
 
 
 def l1_loss_grad(pred: np.ndarray, target: np.ndarray, weight: float = 1.0):

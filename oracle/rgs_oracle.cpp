@@ -490,6 +490,10 @@ template <class T> struct Pipe {
   const T* frozen_vd = nullptr;
   const T* frozen_comp = nullptr;
   bool bwd_termination = false;
+  // FP64 replay of an FP32 forward: the backward's skip and stop decisions are
+  // taken from the FP32 splats with FP32 arithmetic, the gradient values are
+  // computed in T.  Set only on a Pipe<double> built by orc_backward_f64.
+  const std::vector<Proj<float>>* shadow = nullptr;
 
   std::vector<uint8_t> pass1;
   std::vector<T> depth;
@@ -778,6 +782,7 @@ template <class T> struct Pipe {
           const T vo[4] = {go[0], go[1], go[2],
                            (go[3] - ((bg[0] * go[0] + bg[1] * go[1]) + bg[2] * go[2])) * Tf};
           T po[4] = {K<T>(0.0), K<T>(0.0), K<T>(0.0), K<T>(1.0)};
+          float shadow_T = 1.0f;  // FP32 transmittance of the replayed forward (shadow mode)
           for (uint32_t i = s; i < e; ++i) {
             const Proj<T>& sp = proj[isect_cgid[i]];
             const T col[3] = {sp.r, sp.g, sp.bl};
@@ -790,14 +795,35 @@ template <class T> struct Pipe {
               sigma = K<T>(0.5) * (sp.a * dx * dx + K<T>(2.0) * sp.b * dx * dy + sp.c * dy * dy);
             const T gauss = Mth<T>::exp(-sigma);
             const T alpha = mn(K<T>(0.999), sp.o * gauss);
-            if (sigma < K<T>(0.0) || alpha < K<T>(1.0 / 255.0)) continue;
-            const T next_T = po[3] * (K<T>(1.0) - alpha);
             bool last = false;
-            if (mode == 0) {
-              if (next_T <= K<T>(1e-4)) break;
+            bool geo = sp.o * gauss <= K<T>(0.999);  // the alpha clamp gates the geometric terms
+            if (shadow) {
+              // the same decisions as the FP32 backward over the FP32 splats
+              const Proj<float>& fs = (*shadow)[isect_cgid[i]];
+              const float fdx = fs.x - ((float)px + 0.5f), fdy = fs.y - ((float)py + 0.5f);
+              const float fsig = mode == 0 ? 0.5f * (fs.a * fdx * fdx + fs.c * fdy * fdy) + fs.b * fdx * fdy
+                                           : 0.5f * (fs.a * fdx * fdx + 2.0f * fs.b * fdx * fdy + fs.c * fdy * fdy);
+              const float fog = fs.o * rgs_exp(-fsig);
+              const float falpha = std::min(0.999f, fog);
+              geo = fog <= 0.999f;
+              if (fsig < 0.0f || falpha < (float)(1.0 / 255.0)) continue;
+              const float fnext = shadow_T * (1.0f - falpha);
+              if (mode == 0) {
+                if (fnext <= 1e-4f) break;
+              } else {
+                last = fnext < 1e-4f;
+              }
+              shadow_T = fnext;
             } else {
-              last = next_T < K<T>(1e-4);
+              if (sigma < K<T>(0.0) || alpha < K<T>(1.0 / 255.0)) continue;
+              const T nt = po[3] * (K<T>(1.0) - alpha);
+              if (mode == 0) {
+                if (nt <= K<T>(1e-4)) break;
+              } else {
+                last = nt < K<T>(1e-4);
+              }
             }
+            const T next_T = po[3] * (K<T>(1.0) - alpha);
             const T vis = alpha * po[3];
             T vrgb[3];
             for (int c = 0; c < 3; ++c) vrgb[c] = col[c] >= K<T>(0.0) ? vis * vo[c] : K<T>(0.0);
@@ -811,7 +837,7 @@ template <class T> struct Pipe {
             T vcon[3] = {K<T>(0.0), K<T>(0.0), K<T>(0.0)};
             T vxy[2] = {K<T>(0.0), K<T>(0.0)};
             T vat = K<T>(0.0), vref = K<T>(0.0);
-            if (sp.o * gauss <= K<T>(0.999)) {
+            if (geo) {
               vcon[0] = K<T>(0.5) * v_sigma * dx * dx;
               vcon[1] = v_sigma * dx * dy;
               vcon[2] = K<T>(0.5) * v_sigma * dy * dy;
@@ -1001,6 +1027,7 @@ template <class T> struct Pipe {
 // =================================================================== C ABI
 struct orc_render {
   std::vector<float> tf, sh, op;
+  orc_camera cam;
   Pipe<float>* p = nullptr;
   ~orc_render() { delete p; }
 };
@@ -1015,6 +1042,7 @@ orc_render* orc_forward(int64_t n, int sh_degree, const float* transforms, const
   r->tf.assign(transforms, transforms + n * 10);
   r->sh.assign(sh, sh + n * nk * 3);
   r->op.assign(raw_opac, raw_opac + n);
+  r->cam = *cam;
   r->p = new Pipe<float>(*cam);
   Pipe<float>& p = *r->p;
   p.n = n;
@@ -1078,6 +1106,50 @@ int orc_backward(orc_render* r, const float* v_out, int mode, float* v_tf, float
   if (vs_out) std::copy(vs.begin(), vs.end(), vs_out);
   if (ss_out) std::copy(ss.begin(), ss.end(), ss_out);
   p.project_bwd(vs, ss, mode, v_tf, v_sh, v_op, stats);
+  return 0;
+}
+
+int orc_backward_f64(orc_render* r, const float* v_out, int mode, const float* vs_in, double* v_tf, double* v_sh,
+                     double* v_op, double* vs_out) {
+  if (!r || !v_out || (mode != 0 && mode != 1)) return 1;
+  const Pipe<float>& p = *r->p;
+  const int64_t n = p.n;
+  std::vector<double> tf(r->tf.begin(), r->tf.end()), sh(r->sh.begin(), r->sh.end()), op(r->op.begin(), r->op.end());
+  Pipe<double> d(r->cam);
+  d.n = n;
+  d.deg = p.deg;
+  d.nk = p.nk;
+  d.tf = tf.data();
+  d.sh = sh.data();
+  d.op = op.data();
+  for (int i = 0; i < 3; ++i) d.bg[i] = p.bg[i];
+  d.blur = p.blur;
+  d.tw = p.tw;
+  d.tbx = p.tbx;
+  d.tby = p.tby;
+  d.ntiles = p.ntiles;
+  d.order = p.order;
+  d.isect_cgid = p.isect_cgid;
+  d.isect_tile = p.isect_tile;
+  d.offsets = p.offsets;
+  d.proj.resize(p.proj.size());
+  for (size_t i = 0; i < p.proj.size(); ++i) {
+    const Proj<float>& f = p.proj[i];
+    d.proj[i] = Proj<double>{f.x, f.y, f.a, f.b, f.c, f.r, f.g, f.bl, f.o};
+  }
+  d.img.assign(p.img.begin(), p.img.end());
+  d.shadow = &p.proj;
+  const int64_t np = (int64_t)p.v.W * p.v.H;
+  std::vector<double> vo(v_out, v_out + np * 4);
+  std::vector<double> vs, ss;
+  if (vs_in) {
+    vs.assign(vs_in, vs_in + p.proj.size() * 10);
+    ss.assign(p.proj.size() * 5, 0.0);
+  } else {
+    d.raster_bwd(vo.data(), mode, vs, ss);
+  }
+  if (vs_out) std::copy(vs.begin(), vs.end(), vs_out);
+  d.project_bwd(vs, ss, mode, v_tf, v_sh, v_op, nullptr);
   return 0;
 }
 

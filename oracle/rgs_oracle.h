@@ -67,6 +67,16 @@ int orc_backward(orc_render* r, const float* v_out_rgba, int mode,
                  float* screen_stats, float* v_splats_compact,
                  float* screen_splats_compact);
 
+/* FP64 replay of orc_backward over this FP32 forward: the same sorted lists,
+ * projected splats (promoted to double), image, and the FP32 backward's skip
+ * and stop decisions; the gradient arithmetic of K8 and K9 in double.  With
+ * vs_in (V*10 compact splat gradients, FP32) K8 is skipped and only K9 runs
+ * in double on those values.  Measures how much of an FP32 backward's error
+ * is rounding.  Outputs as orc_backward (doubles); any pointer may be NULL. */
+int orc_backward_f64(orc_render* r, const float* v_out_rgba, int mode, const float* vs_in,
+                     double* v_transforms, double* v_sh, double* v_raw_opac,
+                     double* v_splats_compact);
+
 /* ---- L1 loss (engine/loss.rs reconstruction_residual_loss, plain L1) ---- */
 double orc_l1_loss_grad(const float* pred_rgba, const float* target_rgb,
                         int64_t width, int64_t height, float weight,

@@ -606,9 +606,16 @@ __global__ void k_fwd_begin(uint32_t* __restrict__ counters) {
   }
 }
 
-// Step epilogue: count the step as skipped when it did not update.
+// Step epilogue: count the step as skipped when it did not update, and keep
+// a capacity overflow (and the intersection count it needed) in the sticky
+// counters until the host has re-sized.
 __global__ void k_step_end(uint32_t* __restrict__ counters) {
-  if (threadIdx.x == 0 && counters[kCntStepValid] == 0u) counters[kCntSkipped] += 1u;
+  if (threadIdx.x != 0) return;
+  if (counters[kCntStepValid] == 0u) counters[kCntSkipped] += 1u;
+  if (counters[kCntOverflow] != 0u) {
+    counters[kCntOverflowSticky] = 1u;
+    counters[kCntIsectMaxSticky] = max(counters[kCntIsectMaxSticky], counters[kCntIsectMax]);
+  }
 }
 
 // ------------------------------------------------------------------ host side

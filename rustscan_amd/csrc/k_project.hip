@@ -412,10 +412,44 @@ __device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const floa
 // The reference accumulates v_xy a second time into its screen-gradient
 // buffer (rasterize_backwards.wgsl:158-159); the same sum is read from [0..1].
 
+// ---- the covariance chain of the projection backward, in FP64 ----------
+// project_backwards.wgsl:112-150 and helpers.wgsl:56-203,382-427 restated in
+// double.  In FP32 this chain loses the gradient of elongated splats: the
+// conic, the inverse VJP -C*V*C and the products with J, the view rotation
+// and R*S are formed in pixel / world bases, where the large components along
+// a splat's long axis cancel, and the rounding left along that axis is then
+// scaled by the long axis' scale squared (quaternion and mean gradients of a
+// splat with a 1:1000 axis ratio lose every digit).  Double arithmetic keeps
+// the chain within the FP64 result; CDNA4 issues FP64 FMAs at the non-packed
+// FP32 rate and this kernel is bound by memory, not by these operations.
+// Rotation of a unit quaternion (w, x, y, z), helpers.wgsl:56-80, in double; r[c][row].
+__device__ __forceinline__ void quat_rot64(const double q[4], double r[3][3]) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  const double x2 = x * x, y2 = y * y, z2 = z * z;
+  const double xy = x * y, xz = x * z, yz = y * z;
+  const double wx = w * x, wy = w * y, wz = w * z;
+  r[0][0] = 1.0 - 2.0 * (y2 + z2); r[0][1] = 2.0 * (xy + wz); r[0][2] = 2.0 * (xz - wy);
+  r[1][0] = 2.0 * (xy - wz); r[1][1] = 1.0 - 2.0 * (x2 + z2); r[1][2] = 2.0 * (yz + wx);
+  r[2][0] = 2.0 * (xz + wy); r[2][1] = 2.0 * (yz - wx); r[2][2] = 1.0 - 2.0 * (x2 + y2);
+}
+
+// Symmetric 3x3 matrix: xx, yy, zz, xy, xz, yz.
+struct S3 {
+  double xx, yy, zz, xy, xz, yz;
+  __device__ __forceinline__ double at(int i, int j) const {
+    if (i == j) return i == 0 ? xx : (i == 1 ? yy : zz);
+    const int k = i + j;  // 1: xy, 2: xz, 3: yz
+    return k == 1 ? xy : (k == 2 ? xz : yz);
+  }
+};
+
 // grads: SoA planes like the parameters (59 planes at SH3).
 // stats: [7][npad] screen-gradient statistics (project_backwards.wgsl:33-36,106-108), summed over views.
-// visible: [npad] OR over views of the raster visibility (1.0/0.0), written when non-null.
 // accumulate != 0 adds into grads/stats instead of overwriting (per-view launches).
+//
+// Symmetric matrices of the chain (covariances and their gradients) are
+// handled as symmetric: the reference forms them as literal products, whose
+// asymmetric rounding is below FP64's.
 template <int DEG>
 __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P, uint32_t npad, uint32_t n,
                                                      const ViewDev* __restrict__ views, int v0, int nv, float blur,
@@ -426,209 +460,227 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= n) return;
   const float mean[3] = {P[gid], P[npad + gid], P[2 * npad + gid]};
-  const float qu[4] = {P[3 * npad + gid], P[4 * npad + gid], P[5 * npad + gid], P[6 * npad + gid]};
-  const float sc[3] = {rgs_exp(P[7 * npad + gid]), rgs_exp(P[8 * npad + gid]), rgs_exp(P[9 * npad + gid])};
-  const float opac = rgs_sigmoid(P[10 * npad + gid]);
-  const float qn = ((qu[0] * qu[0] + qu[1] * qu[1]) + qu[2] * qu[2]) + qu[3] * qu[3];
-  const float is = 1.0f / rgs_sqrt(qn);
-  const float q[4] = {qu[0] * is, qu[1] * is, qu[2] * is, qu[3] * is};
-  Geom g;
-  make_geom(q[0], q[1], q[2], q[3], sc, g);
-
-  float gmean[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f}, gop = 0.f;
-  float gsh[NK * 3];
-#pragma unroll
-  for (int i = 0; i < NK * 3; ++i) gsh[i] = 0.f;
-  float st[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  bool any = false;
-
-  for (int v = v0; v < v0 + nv; ++v) {
-    const size_t inst = (size_t)v * n + gid;
-    if (keys_by_inst[inst] == 0xffffffffu) continue;  // culled in this view
-    float4* ap = reinterpret_cast<float4*>(acc + inst * kAcc);
-    const float4 a3 = ap[3];
-    // coverage 0: the rasterizer never flushed into this line, which is
-    // still all zero, and every output term is zero
-    if (a3.x == 0.0f) continue;
-    const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2];
-    // consume and clear so the accumulator starts at zero next step
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    ap[0] = z4; ap[1] = z4; ap[2] = z4; ap[3] = z4;
-    any = true;
-    const ViewDev& V = views[v];
-    const float vxy[2] = {a0.x, a0.y};
-    const float vcon[3] = {a0.z, a0.w, a1.x};
-    const float vcol[3] = {a1.y, a1.z, a1.w};
-    const float vca = a2.x;
-    // screen statistics [0..3]
-    st[0] += a0.x; st[1] += a0.y; st[2] += a2.z; st[3] += a2.w;
-    // SH
-    const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
-    const float dd = (dx * dx + dy * dy) + dz * dz;
-    const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
-    sh_vjp_acc<DEG>(dx * rs, dy * rs, dz * rs, vcol, gsh);
-    const float mcx = ((V.vm[0] * mean[0] + V.vm[4] * mean[1]) + V.vm[8] * mean[2]) + V.vm[12];
-    const float mcy = ((V.vm[1] * mean[0] + V.vm[5] * mean[1]) + V.vm[9] * mean[2]) + V.vm[13];
-    const float mcz = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
-    if (mcz < RGS_F(0.01)) continue;
-    Proj2 pc;
-    project_cov(g, V, mcx, mcy, mcz, pc);
-    M2 c2 = pc.cov2d;
-    const float det_raw = fmaxf_(det2(c2), 0.0f);
-    c2.c00 = c2.c00 + blur;
-    c2.c11 = c2.c11 + blur;
-    const float det_end = fmaxf_(det2(c2), RGS_F(1e-12));
-    const float comp = det_raw <= 0.0f ? 0.0f : rgs_sqrt(det_raw / det_end);
-    const float abs_score = rgs_sqrt(a2.z * a2.z + a2.w * a2.w);
-    st[4] += a3.x * abs_score;
-    st[5] += a3.x;
-    st[6] += a3.x * mcz;
-    gop += comp * vca * opac * (1.0f - opac);
-    // v_cov2d = -Cinv * V * Cinv (inverse2x2_vjp) with V = [[vc0, vc1/2], [vc1/2, vc2]]
-    const M2 ci = inverse2x2(c2);
-    const float h = 0.5f * vcon[1];
-    // (-ci) * V
-    const float t00 = -ci.c00 * vcon[0] + -ci.c10 * h, t01 = -ci.c01 * vcon[0] + -ci.c11 * h;
-    const float t10 = -ci.c00 * h + -ci.c10 * vcon[2], t11 = -ci.c01 * h + -ci.c11 * vcon[2];
-    // (...) * ci
-    M2 vc;
-    vc.c00 = t00 * ci.c00 + t10 * ci.c01;
-    vc.c01 = t01 * ci.c00 + t11 * ci.c01;
-    vc.c10 = t00 * ci.c10 + t10 * ci.c11;
-    vc.c11 = t01 * ci.c10 + t11 * ci.c11;
-    // persp_proj_vjp (helpers.wgsl:382-427)
-    const float x = mcx, y = mcy, zz = mcz;
-    const float rz = 1.0f / fmaxf_(fabsf(zz), RGS_F(0.01));
-    const float rz2 = rz * rz;
-    float vm0 = V.fx * rz * vxy[0];
-    float vm1 = V.fy * rz * vxy[1];
-    float vm2 = -(V.fx * x * vxy[0] + V.fy * y * vxy[1]) * rz2;
-    // v_J = vc * J * cc^T + vc^T * J * cc   (cc symmetric up to rounding: use both literally)
-    const float J[3][2] = {{pc.jx, 0.0f}, {0.0f, pc.jy}, {pc.j20, pc.j21}};
-    float vJ[3][2];
-    {
-      float VJ[3][2], VtJ[3][2];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        VJ[c][0] = vc.c00 * J[c][0] + vc.c10 * J[c][1];
-        VJ[c][1] = vc.c01 * J[c][0] + vc.c11 * J[c][1];
-        VtJ[c][0] = vc.c00 * J[c][0] + vc.c01 * J[c][1];
-        VtJ[c][1] = vc.c10 * J[c][0] + vc.c11 * J[c][1];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          // (VJ * cc^T)[c][r] = sum_k VJ[k][r] * cc[k][c];  (VtJ * cc)[c][r] = sum_k VtJ[k][r] * cc[c][k]
-          const float p1 = (VJ[0][r] * pc.cc[0][c] + VJ[1][r] * pc.cc[1][c]) + VJ[2][r] * pc.cc[2][c];
-          const float p2 = (VtJ[0][r] * pc.cc[c][0] + VtJ[1][r] * pc.cc[c][1]) + VtJ[2][r] * pc.cc[c][2];
-          vJ[c][r] = p1 + p2;
-        }
-    }
-    if (x * rz <= V.lim_px && x * rz >= V.lim_nx) {
-      vm0 = vm0 + -V.fx * rz2 * vJ[2][0];
-      vm2 = vm2 + 2.0f * V.fx * x * rz2 * rz * vJ[2][0];
-    } else {
-      const float cxl = clampf_(x * rz, V.lim_nx, V.lim_px);
-      vm2 = vm2 + V.fx * cxl * rz2 * vJ[2][0];
-    }
-    if (y * rz <= V.lim_py && y * rz >= V.lim_ny) {
-      vm1 = vm1 + -V.fy * rz2 * vJ[2][1];
-      vm2 = vm2 + 2.0f * V.fy * y * rz2 * rz * vJ[2][1];
-    } else {
-      const float cyl = clampf_(y * rz, V.lim_ny, V.lim_py);
-      vm2 = vm2 + V.fy * cyl * rz2 * vJ[2][1];
-    }
-    vm2 = vm2 + (-V.fx * rz2 * vJ[0][0] - V.fy * rz2 * vJ[1][1]);
-    // v_cov_cam = J^T * vc * J  (3x3, [c][r] = sum_k,l J[r][k]... written out)
-    float vcc[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        // (J^T vc)[col k][row r] = J[r][0]*vc[k][0] + J[r][1]*vc[k][1]; then * J: sum_k (J^T vc)[k][r] * J[c][k]
-        const float a0_ = J[r][0] * vc.c00 + J[r][1] * vc.c01;
-        const float a1_ = J[r][0] * vc.c10 + J[r][1] * vc.c11;
-        vcc[c][r] = a0_ * J[c][0] + a1_ * J[c][1];
-      }
-    // v_cov3d = R^T * vcc * R, R[c][r] = vm[c*4+r]
-    float T1[3][3], vc3[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r)  // (R^T vcc)[c][r] = sum_k R^T[k][r] vcc[c][k] = sum_k R[r][k] vcc[c][k]
-        T1[c][r] = (V.vm[r * 4 + 0] * vcc[c][0] + V.vm[r * 4 + 1] * vcc[c][1]) + V.vm[r * 4 + 2] * vcc[c][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r)  // (T1 R)[c][r] = sum_k T1[k][r] R[c][k]
-        vc3[c][r] = (T1[0][r] * V.vm[c * 4 + 0] + T1[1][r] * V.vm[c * 4 + 1]) + T1[2][r] * V.vm[c * 4 + 2];
-    // v_m = (vc3 + vc3^T) * m
-    float vm_[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        float s = 0.f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) s += (vc3[k][r] + vc3[r][k]) * g.m[c][k];
-        vm_[c][r] = s;
-      }
-    float rot[3][3];
-    quat_rot(q[0], q[1], q[2], q[3], rot);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float vs = (rot[c][0] * vm_[c][0] + rot[c][1] * vm_[c][1]) + rot[c][2] * vm_[c][2];
-      gls[c] += vs * sc[c];
-    }
-    // quat_to_mat3_vjp(q, v_m * S)
-    float vr[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) vr[c][r] = vm_[c][r] * sc[c];
-    const float w = q[0], qx = q[1], qy = q[2], qz = q[3];
-    float dq[4];
-    dq[0] = 2.0f * (qx * (vr[1][2] - vr[2][1]) + qy * (vr[2][0] - vr[0][2]) + qz * (vr[0][1] - vr[1][0]));
-    dq[1] = 2.0f * (-2.0f * qx * (vr[1][1] + vr[2][2]) + qy * (vr[0][1] + vr[1][0]) + qz * (vr[0][2] + vr[2][0]) +
-                    w * (vr[1][2] - vr[2][1]));
-    dq[2] = 2.0f * (qx * (vr[0][1] + vr[1][0]) - 2.0f * qy * (vr[0][0] + vr[2][2]) + qz * (vr[1][2] + vr[2][1]) +
-                    w * (vr[2][0] - vr[0][2]));
-    dq[3] = 2.0f * (qx * (vr[0][2] + vr[2][0]) + qy * (vr[1][2] + vr[2][1]) - 2.0f * qz * (vr[0][0] + vr[1][1]) +
-                    w * (vr[0][1] - vr[1][0]));
-    // normalize_vjp(qu): (|q|^2 I - q q^T) / |q|^3, symmetric
-    {
-      const float len = rgs_sqrt(fmaxf_(qn, RGS_F(1e-20)));
-      const float sN = 1.0f / (len * qn);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float s = 0.f;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float e = (k == r) ? (qn - qu[k] * qu[k]) : -(qu[k] * qu[r]);
-          s += e * sN * dq[k];
-        }
-        gq[r] += s;
-      }
-    }
-    // v_mean = R^T v_mean_c
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-      gmean[r] += (V.vm[r * 4 + 0] * vm0 + V.vm[r * 4 + 1] * vm1) + V.vm[r * 4 + 2] * vm2;
-  }
-
-  // write (or accumulate) the gradient planes
   auto out = [&](int plane, float val) {
     float* p = grads + (size_t)plane * npad + gid;
     *p = accumulate ? (*p + val) : val;
   };
-  if (!any && accumulate) return;
+
+  // Pass 1: the SH colour VJP (FP32: the basis is bounded and its products
+  // do not cancel), with the view direction from the unperturbed mean and no
+  // mean gradient through it (project_backwards.wgsl:68-79).  Its planes are
+  // written before pass 2 so the two passes' accumulators are never live
+  // together.
+  bool any = false;
+  {
+    float gsh[NK * 3];
+#pragma unroll
+    for (int i = 0; i < NK * 3; ++i) gsh[i] = 0.f;
+    for (int v = v0; v < v0 + nv; ++v) {
+      const size_t inst = (size_t)v * n + gid;
+      if (keys_by_inst[inst] == 0xffffffffu) continue;  // culled in this view
+      const float4* ap = reinterpret_cast<const float4*>(acc + inst * kAcc);
+      // coverage 0: the rasterizer never flushed into this line, which is
+      // still all zero, and every output term is zero
+      if (ap[3].x == 0.0f) continue;
+      const float4 a1 = ap[1];
+      any = true;
+      const ViewDev& V = views[v];
+      const float vcol[3] = {a1.y, a1.z, a1.w};
+      const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
+      const float dd = (dx * dx + dy * dy) + dz * dz;
+      const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
+      sh_vjp_acc<DEG>(dx * rs, dy * rs, dz * rs, vcol, gsh);
+    }
+    if (!any && accumulate) return;
+#pragma unroll
+    for (int i = 0; i < NK * 3; ++i) out(11 + i, gsh[i]);
+  }
+
+  // Pass 2: opacity, screen statistics and the covariance chain in FP64;
+  // consumes and clears the accumulator lines.
+  const float opac = rgs_sigmoid(P[10 * npad + gid]);
+  const double qu[4] = {(double)P[3 * npad + gid], (double)P[4 * npad + gid], (double)P[5 * npad + gid],
+                        (double)P[6 * npad + gid]};
+  // quaternion normalisation and scales (project_backwards.wgsl:57-63)
+  const double qn = qu[0] * qu[0] + qu[1] * qu[1] + qu[2] * qu[2] + qu[3] * qu[3];
+  const double qlen = sqrt(fmax(qn, 1e-20));
+  double q[4];
+  {
+    const double qis = 1.0 / qlen;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = qu[i] * qis;
+  }
+  double rot[3][3];
+  quat_rot64(q, rot);
+  double s2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double sc = exp((double)P[(7 + c) * npad + gid]);
+    s2[c] = sc * sc;
+  }
+  float gmean[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f};
+  float gop = 0.f;
+  float st[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int v = v0; v < v0 + nv; ++v) {
+    const size_t inst = (size_t)v * n + gid;
+    if (keys_by_inst[inst] == 0xffffffffu) continue;
+    float4* ap = reinterpret_cast<float4*>(acc + inst * kAcc);
+    const float4 a3 = ap[3];
+    if (a3.x == 0.0f) continue;
+    const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2];
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    ap[0] = z4; ap[1] = z4; ap[2] = z4; ap[3] = z4;
+    const ViewDev& V = views[v];
+    const float vca = a2.x;
+    // screen statistics [0..3]
+    st[0] += a0.x; st[1] += a0.y; st[2] += a2.z; st[3] += a2.w;
+    const float mcz_f = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
+    if (mcz_f < RGS_F(0.01)) continue;
+    {
+      const float abs_score = rgs_sqrt(a2.z * a2.z + a2.w * a2.w);
+      st[4] += a3.x * abs_score;
+      st[5] += a3.x;
+      st[6] += a3.x * mcz_f;
+    }
+    // B = Rv * rot (the Gaussian's axes in camera space); cc = B diag(s^2) B^T
+    double B[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        B[c][r] = (double)V.vm[r] * rot[c][0] + (double)V.vm[4 + r] * rot[c][1] + (double)V.vm[8 + r] * rot[c][2];
+    S3 cc;
+    cc.xx = B[0][0] * B[0][0] * s2[0] + B[1][0] * B[1][0] * s2[1] + B[2][0] * B[2][0] * s2[2];
+    cc.yy = B[0][1] * B[0][1] * s2[0] + B[1][1] * B[1][1] * s2[1] + B[2][1] * B[2][1] * s2[2];
+    cc.zz = B[0][2] * B[0][2] * s2[0] + B[1][2] * B[1][2] * s2[1] + B[2][2] * B[2][2] * s2[2];
+    cc.xy = B[0][0] * B[0][1] * s2[0] + B[1][0] * B[1][1] * s2[1] + B[2][0] * B[2][1] * s2[2];
+    cc.xz = B[0][0] * B[0][2] * s2[0] + B[1][0] * B[1][2] * s2[1] + B[2][0] * B[2][2] * s2[2];
+    cc.yz = B[0][1] * B[0][2] * s2[0] + B[1][1] * B[1][2] * s2[1] + B[2][1] * B[2][2] * s2[2];
+    // camera-space mean
+    double mc[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      mc[r] = (double)V.vm[r] * mean[0] + (double)V.vm[4 + r] * mean[1] + (double)V.vm[8 + r] * mean[2] +
+              (double)V.vm[12 + r];
+    // J = [[jx, 0, j20], [0, jy, j21]] with the EWA clip (helpers.wgsl:138-155)
+    const double fx = V.fx, fy = V.fy;
+    const double lpx = V.lim_px, lpy = V.lim_py, lnx = V.lim_nx, lny = V.lim_ny;
+    const double rzj = 1.0 / fmax(mc[2], 0.01);
+    const double jx = fx * rzj, jy = fy * rzj;
+    const double j20 = -jx * fmin(fmax(mc[0] * rzj, lnx), lpx), j21 = -jy * fmin(fmax(mc[1] * rzj, lny), lpy);
+    // cov2d = J cc J^T
+    const double c2xx = jx * (jx * cc.xx + j20 * cc.xz) + j20 * (jx * cc.xz + j20 * cc.zz);
+    const double c2yy = jy * (jy * cc.yy + j21 * cc.yz) + j21 * (jy * cc.yz + j21 * cc.zz);
+    const double c2xy = jx * (jy * cc.xy + j21 * cc.xz) + j20 * (jy * cc.yz + j21 * cc.zz);
+    // blur compensation (helpers.wgsl:170-183); its derivative is not taken (project_backwards.wgsl:110)
+    const double det_raw = fmax(c2xx * c2yy - c2xy * c2xy, 0.0);
+    const double bxx = c2xx + (double)blur, byy = c2yy + (double)blur;
+    const double det_end = fmax(bxx * byy - c2xy * c2xy, 1e-12);
+    const double comp = det_raw <= 0.0 ? 0.0 : sqrt(det_raw / det_end);
+    gop += (float)comp * vca * opac * (1.0f - opac);
+    // conic = inverse2x2(cov2d) with the +1e-6 regulariser (helpers.wgsl:185-198)
+    double cixx, ciyy, cixy;
+    {
+      const double r00 = bxx + 1e-6, r11 = byy + 1e-6;
+      const double det = r00 * r11 - c2xy * c2xy;
+      const double inv = det <= 1e-10 ? 0.0 : 1.0 / det;
+      cixx = r11 * inv; ciyy = r00 * inv; cixy = -c2xy * inv;
+    }
+    // v_cov2d = -C V C with V = [[vc0, vc1/2], [vc1/2, vc2]] (helpers.wgsl:200-202)
+    double vxx, vyy, vxy;
+    {
+      const double w0 = a0.z, wh = 0.5 * (double)a0.w, w2 = a1.x;
+      const double t00 = cixx * w0 + cixy * wh, t01 = cixy * w0 + ciyy * wh;  // (C V) column 0
+      const double t10 = cixx * wh + cixy * w2, t11 = cixy * wh + ciyy * w2;  // (C V) column 1
+      vxx = -(t00 * cixx + t10 * cixy);
+      vxy = -(t01 * cixx + t11 * cixy);
+      vyy = -(t01 * cixy + t11 * ciyy);
+    }
+    // persp_proj_vjp (helpers.wgsl:382-427); v_J = 2 vc J cc for symmetric vc and cc
+    const double x = mc[0], y = mc[1];
+    const double rz = 1.0 / fmax(fabs(mc[2]), 0.01);
+    const double rz2 = rz * rz;
+    double vm0 = fx * rz * (double)a0.x;
+    double vm1 = fy * rz * (double)a0.y;
+    double vm2 = -(fx * x * (double)a0.x + fy * y * (double)a0.y) * rz2;
+    // (vc J) columns: vc (jx, 0), vc (0, jy), vc (j20, j21)
+    const double p00 = vxx * jx, p01 = vxy * jx;
+    const double p10 = vxy * jy, p11 = vyy * jy;
+    const double p20 = vxx * j20 + vxy * j21, p21 = vxy * j20 + vyy * j21;
+    const double vJ00 = 2.0 * (p00 * cc.xx + p10 * cc.xy + p20 * cc.xz);
+    const double vJ11 = 2.0 * (p01 * cc.xy + p11 * cc.yy + p21 * cc.yz);
+    const double vJ20 = 2.0 * (p00 * cc.xz + p10 * cc.yz + p20 * cc.zz);
+    const double vJ21 = 2.0 * (p01 * cc.xz + p11 * cc.yz + p21 * cc.zz);
+    if (x * rz <= lpx && x * rz >= lnx) {
+      vm0 += -fx * rz2 * vJ20;
+      vm2 += 2.0 * fx * x * rz2 * rz * vJ20;
+    } else {
+      vm2 += fx * fmin(fmax(x * rz, lnx), lpx) * rz2 * vJ20;
+    }
+    if (y * rz <= lpy && y * rz >= lny) {
+      vm1 += -fy * rz2 * vJ21;
+      vm2 += 2.0 * fy * y * rz2 * rz * vJ21;
+    } else {
+      vm2 += fy * fmin(fmax(y * rz, lny), lpy) * rz2 * vJ21;
+    }
+    vm2 += -fx * rz2 * vJ00 - fy * rz2 * vJ11;
+    // v_mean = Rv^T v_mean_c
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      gmean[r] += (float)((double)V.vm[r * 4] * vm0 + (double)V.vm[r * 4 + 1] * vm1 + (double)V.vm[r * 4 + 2] * vm2);
+    // v_cov_cam = J^T vc J (symmetric)
+    S3 vcc;
+    vcc.xx = jx * jx * vxx;
+    vcc.yy = jy * jy * vyy;
+    vcc.xy = jx * jy * vxy;
+    vcc.xz = jx * p20;
+    vcc.yz = jy * p21;
+    vcc.zz = j20 * p20 + j21 * p21;
+    // In the Gaussian's frame, G = B^T vcc B (G[a][b] = B[a] . vcc B[b]).  With
+    // v_cov3d = Rv^T vcc Rv and m = rot diag(s):
+    //   v_m[c] = 2 v_cov3d rot[c] s_c,  v_scale[c] = rot[c] . v_m[c] = 2 G[c][c] s_c,
+    //   v_rot[c] = v_m[c] s_c = 2 s_c^2 sum_b G[b][c] rot[b]
+    double G[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double vb[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) vb[r] = vcc.at(r, 0) * B[b][0] + vcc.at(r, 1) * B[b][1] + vcc.at(r, 2) * B[b][2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) G[a][b] = B[a][0] * vb[0] + B[a][1] * vb[1] + B[a][2] * vb[2];
+    }
+    // log-scale: dL/dlog s = s dL/ds
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gls[c] += (float)(2.0 * G[c][c] * s2[c]);
+    double vr[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        vr[c][r] = 2.0 * s2[c] * (G[0][c] * rot[0][r] + G[1][c] * rot[1][r] + G[2][c] * rot[2][r]);
+    // quat_to_mat3_vjp (helpers.wgsl:82-103)
+    const double w = q[0], qx = q[1], qy = q[2], qz = q[3];
+    double dq[4];
+    dq[0] = 2.0 * (qx * (vr[1][2] - vr[2][1]) + qy * (vr[2][0] - vr[0][2]) + qz * (vr[0][1] - vr[1][0]));
+    dq[1] = 2.0 * (-2.0 * qx * (vr[1][1] + vr[2][2]) + qy * (vr[0][1] + vr[1][0]) + qz * (vr[0][2] + vr[2][0]) +
+                   w * (vr[1][2] - vr[2][1]));
+    dq[2] = 2.0 * (qx * (vr[0][1] + vr[1][0]) - 2.0 * qy * (vr[0][0] + vr[2][2]) + qz * (vr[1][2] + vr[2][1]) +
+                   w * (vr[2][0] - vr[0][2]));
+    dq[3] = 2.0 * (qx * (vr[0][2] + vr[2][0]) + qy * (vr[1][2] + vr[2][1]) - 2.0 * qz * (vr[0][0] + vr[1][1]) +
+                   w * (vr[0][1] - vr[1][0]));
+    // normalize_vjp(qu) (helpers.wgsl:105-119): (|q|^2 I - q q^T) / |q|^3
+    {
+      const double sN = 1.0 / (qlen * qn);
+      const double qd = qu[0] * dq[0] + qu[1] * dq[1] + qu[2] * dq[2] + qu[3] * dq[3];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) gq[r] += (float)((qn * dq[r] - qu[r] * qd) * sN);
+    }
+  }
+
   out(0, gmean[0]); out(1, gmean[1]); out(2, gmean[2]);
   out(3, gq[0]); out(4, gq[1]); out(5, gq[2]); out(6, gq[3]);
   out(7, gls[0]); out(8, gls[1]); out(9, gls[2]);
   out(10, gop);
-#pragma unroll
-  for (int i = 0; i < NK * 3; ++i) out(11 + i, gsh[i]);
   if (stats) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) {

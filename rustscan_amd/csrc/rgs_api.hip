@@ -145,6 +145,12 @@ struct rgs_ctx {
   // host mirrors
   uint32_t* h_counters = nullptr;
   float* h_fscal = nullptr;
+  // counters block as the last completed training step left it: every step
+  // ends with a device-to-host copy into this pinned block (a node of the
+  // captured graph too), read by the next call after step_done
+  uint32_t* h_step = nullptr;
+  hipEvent_t step_done = nullptr;
+  bool step_pending = false;
   std::map<std::vector<uint32_t>, rgs_raster_cfg> tuned;
   // per-stage timing
   bool timing = false;
@@ -546,7 +552,18 @@ int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cf
   }
   enqueue_adam(c, s, &cfg->adam, nv, cfg->check_loss);
   launch_step_end(c->stream, c->ws.counters.as<uint32_t>());
+  HIPCHK(c, hipMemcpyAsync(c->h_step, c->ws.counters.p, sizeof(uint32_t) * kCntNum, hipMemcpyDeviceToHost,
+                           c->stream));
   return RGS_OK;
+}
+
+// The sticky overflow counters were handled: clear them on the device (in
+// stream order) and in the host copy.
+void clear_overflow_sticky(rgs_ctx* c) {
+  launch_set_u32(c->stream, c->ws.counters.as<uint32_t>() + kCntOverflowSticky, 0u);
+  launch_set_u32(c->stream, c->ws.counters.as<uint32_t>() + kCntIsectMaxSticky, 0u);
+  c->h_step[kCntOverflowSticky] = 0u;
+  c->h_step[kCntIsectMaxSticky] = 0u;
 }
 
 int read_counters(rgs_ctx* c) {
@@ -663,11 +680,17 @@ int rgs_ctx_create(int device, rgs_ctx** out) {
   }
   hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
   if (hipHostMalloc((void**)&c->h_counters, sizeof(uint32_t) * kCntNum, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_fscal, sizeof(float) * kFNum, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_fscal, sizeof(float) * kFNum, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_step, sizeof(uint32_t) * kCntNum, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->step_done, hipEventDisableTiming) != hipSuccess) {
+    if (c->h_counters) hipHostFree(c->h_counters);
+    if (c->h_fscal) hipHostFree(c->h_fscal);
+    if (c->h_step) hipHostFree(c->h_step);
     hipStreamDestroy(c->stream);
     delete c;
     return RGS_ERR_HIP;
   }
+  std::memset(c->h_step, 0, sizeof(uint32_t) * kCntNum);
   *out = c;
   return RGS_OK;
 }
@@ -685,6 +708,8 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   for (auto& b : w.own_img) release(b);
   hipHostFree(c->h_counters);
   hipHostFree(c->h_fscal);
+  hipHostFree(c->h_step);
+  hipEventDestroy(c->step_done);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -983,6 +1008,21 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
   set_lr(s, &cfg->adam, lr.data());
   HIPCHK(c, hipMemcpyAsync(s->lr.p, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice, c->stream));
   const StepPlanKey key = plan_key(c, s, vs, cfg->mode);
+  // An asynchronous step leaves its counters for the next call.  Waiting for
+  // the previous step here keeps one step in flight, so a capacity overflow
+  // (which skips that step's update) is seen before the next step is
+  // enqueued: the plan is re-sized and at most one step is lost.
+  if (c->step_pending) {
+    HIPCHK(c, hipEventSynchronize(c->step_done));
+    c->step_pending = false;
+  }
+  if (c->h_step[kCntOverflowSticky]) {
+    const size_t need = (size_t)c->h_step[kCntIsectMaxSticky];
+    clear_overflow_sticky(c);
+    if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, need * 5 / 4 + 1024))) return rc;
+    c->sized.clear();  // every plan re-sizes (and re-groups its views) on its next call
+    c->groups.clear();
+  }
   if (!c->sized[key]) {
     if ((rc = size_for(c, s, vs, cfg->mode, blur, &c->groups[key]))) return rc;
     c->sized[key] = true;
@@ -1019,12 +1059,17 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
     } else {
       if ((rc = enqueue_step(c, s, vs, cfg, groups))) return rc;
     }
-    if (!loss_out) break;
+    if (!loss_out) {
+      HIPCHK(c, hipEventRecord(c->step_done, c->stream));
+      c->step_pending = true;
+      break;
+    }
     // synchronous call: check the step and retry after growing on overflow
     if ((rc = read_counters(c))) return rc;
     if (c->h_counters[kCntOverflow] == 0) {
       for (int v = 0; v < nv; ++v)
         loss_out[v] = cfg->loss_weight * c->h_fscal[kFLossBase + v] / (float)(3ull * vs.v[v].W * vs.v[v].H);
+      if (c->h_step[kCntOverflowSticky]) clear_overflow_sticky(c);  // an overflowed attempt was redone
       break;
     }
     if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, (size_t)c->h_counters[kCntIsectMax] * 5 / 4 + 1024)))

@@ -54,9 +54,10 @@ enum Counter : int {
   kCntQueueFwd = 3,     // persistent raster work counters
   kCntQueueBwd = 4,
   kCntStepValid = 5,    // 0 when the step must not update parameters
-  kCntUnused6 = 6,
+  kCntOverflowSticky = 6,  // set by a step that overflowed; cleared by the host once it has re-sized
   kCntIsectMax = 7,     // max intersections of any forward in this step
   kCntSkipped = 8,      // steps whose update was skipped
+  kCntIsectMaxSticky = 9,  // largest kCntIsectMax of the overflowed steps since the last clear
   kCntNum = 16,
 };
 

@@ -122,3 +122,50 @@ def test_train_step_grows_its_capacity(orc):
         ds.close()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_async_steps_recover_from_overflow(orc, graph):
+    """Asynchronous steps (no loss read back) on a context sized for the
+    sparse scene, then given the dense one: the first dense step overflows
+    and is skipped, the next call sees the overflow left by that step, grows
+    and re-plans, and every later step trains; the following step's
+    gradients equal the oracle's at the parameters it starts from."""
+    import torch
+
+    from rustscan_amd.rasterizer import Context, DeviceSplats, Trainer, training_optimizer
+
+    cam, sparse, dense = sparse_and_dense()
+    target = scene.Rng(75).uniform(240 * 320 * 3).reshape(240, 320, 3).astype(np.float32)
+    tgt = [torch.from_numpy(target).cuda()]
+    ctx = Context(0)
+    try:
+        ctx.set_raster(16, 256, True)
+        tr = Trainer(ctx, training_optimizer(1, 1.6e-4, 1e-3, 5e-3, 5e-2, 2.5e-3, 1.25e-4))
+        ds = DeviceSplats(ctx, sparse)
+        tr.train_step(ds, [cam], tgt, batched=True, graph=graph, read_loss=True)  # plans for the sparse scene
+        ds.upload(dense)
+        before = ctx.skipped_steps()
+        say(f"async dense steps after sizing for the sparse scene (graph={graph})")
+        start = ds.download()
+        for _ in range(5):
+            tr.train_step(ds, [cam], tgt, batched=True, graph=graph)
+        ctx.sync()
+        skipped = ctx.skipped_steps() - before
+        say(f"  skipped {skipped} of 5 asynchronous steps")
+        assert skipped <= 1
+        assert not np.array_equal(ds.download().transforms, start.transforms)
+        cur = ds.download()
+        losses = tr.train_step(ds, [cam], tgt, batched=True, graph=graph, read_loss=True)
+        assert ctx.skipped_steps() - before == skipped
+        vt, vs, vo, _ = ds.grads()
+        r = orc.Render(cur, cam)
+        ref_loss, v = orc.l1_loss_grad(r.image(), target)
+        g = r.backward(v, 0)
+        np.testing.assert_allclose(losses, [ref_loss], rtol=1e-5)
+        errs = [rel_l2(vt, g["v_transforms"]), rel_l2(vs, g["v_sh"]), rel_l2(vo, g["v_raw_opac"])]
+        say("  grad rel-L2 " + " ".join(f"{e:.2e}" for e in errs))
+        assert max(errs) <= 1e-3
+        ds.close()
+    finally:
+        ctx.close()

@@ -79,7 +79,10 @@ def test_forward_config2_full(hip_ctx, orc):
     check_forward(hip_ctx, orc, sc, cams[0])
 
 
-def grads_vs_oracle(hip_ctx, orc, sc, cam, target, tile=16, batch=256, tol=1e-3):
+def grads_vs_oracle(hip_ctx, orc, sc, cam, target, tile=16, batch=256, tol=1e-3, reference="f32"):
+    """Per-group gradients against the oracle's FP32 backward, or (reference
+    "f64") against its FP64 replay of the same forward; the FP32 oracle's own
+    distance to the FP64 replay is printed beside."""
     import torch
 
     from rustscan_amd.rasterizer import l1_loss_grad, render_backward
@@ -92,6 +95,15 @@ def grads_vs_oracle(hip_ctx, orc, sc, cam, target, tile=16, batch=256, tol=1e-3)
     np.testing.assert_array_equal(v_out.cpu().numpy(), ref_vout)
     vt, vs, vo, st = render_backward(hip_ctx, ds, [v_out])
     ref = r.backward(ref_vout, mode=0)
+    if reference == "f64":
+        r64 = r.backward_f64(ref_vout, mode=0)
+        f32_vs_f64 = {
+            "means": rel_l2(ref["v_transforms"][:, 0:3], r64["v_transforms"][:, 0:3]),
+            "quats": rel_l2(ref["v_transforms"][:, 3:7], r64["v_transforms"][:, 3:7]),
+            "log_scales": rel_l2(ref["v_transforms"][:, 7:10], r64["v_transforms"][:, 7:10]),
+        }
+        say("  FP32 oracle vs FP64 replay " + " ".join(f"{k}={v:.2e}" for k, v in f32_vs_f64.items()))
+        ref = dict(ref, v_transforms=r64["v_transforms"], v_sh=r64["v_sh"], v_raw_opac=r64["v_raw_opac"])
     errs = {
         "means": rel_l2(vt[:, 0:3], ref["v_transforms"][:, 0:3]),
         "quats": rel_l2(vt[:, 3:7], ref["v_transforms"][:, 3:7]),
@@ -176,10 +188,17 @@ def test_batched_forward_matches_oracle_per_view(hip_ctx, orc):
 # ---- needle-like splats: the compositing kernels skip a splat for an 8x8
 # pixel block only when the conic's minimum over the block provably exceeds
 # the alpha threshold; near-degenerate conics sit at the edge of that bound.
-# The images (and binning) must still equal the oracle's.  Gradients are not
-# compared on this scene: some needles project to conics with ac - b^2 near 0,
-# whose inverse makes the float gradients depend on summation order far
-# beyond the 1e-3 tolerance (the oracle itself does not pin them).
+# The images (and binning) must still equal the oracle's.
+#
+# Their gradients are compared with the oracle's backward replayed in FP64
+# over the same FP32 forward (same lists, splats and skip/stop decisions).
+# The FP32 restatement of the reference's projection backward is not a usable
+# reference here: on this scene it is 4.5e-2 (means) and 1.8e-1 (quaternions)
+# rel-L2 away from the FP64 replay, while the FP32 compositing backward
+# followed by an FP64 projection backward is within 1.5e-4 of it
+# (tests/test_oracle_conditioning.py).  The kernel runs the projection
+# backward's covariance chain in FP64, so it is held to 1e-3 of the FP64
+# replay on every group.
 
 
 def needle_scene():
@@ -193,6 +212,14 @@ def test_forward_needle_splats(hip_ctx, orc, tile, batch):
     sc, cam = needle_scene()
     say(f"forward needle splats, tile {tile} batch {batch}")
     check_forward(hip_ctx, orc, sc, cam, tile=tile, batch=batch)
+
+
+@pytest.mark.parametrize("tile,batch", [(16, 256), (32, 256)])
+def test_backward_needle_splats(hip_ctx, orc, tile, batch):
+    sc, cam = needle_scene()
+    target = scene.Rng(93).uniform(cam.width * cam.height * 3).reshape(cam.height, cam.width, 3).astype(np.float32)
+    say(f"backward needle splats vs the FP64 replay, tile {tile} batch {batch}")
+    grads_vs_oracle(hip_ctx, orc, sc, cam, target, tile=tile, batch=batch, reference="f64")
 
 
 # ---- the fast paths against their plain forms: the packed tile sort and the

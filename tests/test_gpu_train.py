@@ -185,8 +185,10 @@ def test_nonfinite_loss_skips_update(hip_ctx, setup):
     assert not np.array_equal(ds.download().transforms, sc.transforms)
 
 
-@pytest.mark.parametrize("tile,batch", [(8, 128), (8, 512), (16, 128), (16, 512), (32, 128), (32, 256), (32, 512)])
-def test_raster_configs_vs_oracle(hip_ctx, orc, tile, batch):
+@pytest.mark.parametrize("tile,batch,persistent", [(8, 128, True), (8, 256, True), (8, 512, True), (16, 128, True),
+                                                    (16, 512, True), (32, 128, True), (32, 256, True),
+                                                    (32, 512, True), (16, 256, False), (32, 256, False)])
+def test_raster_configs_vs_oracle(hip_ctx, orc, tile, batch, persistent):
     import torch
 
     from rustscan_amd.rasterizer import DeviceSplats, l1_loss_grad, render_backward, render_forward
@@ -194,7 +196,7 @@ def test_raster_configs_vs_oracle(hip_ctx, orc, tile, batch):
     sc = scene.gaussian_scene(16384, seed=71, sh_degree=1)
     cam = scene.ring_cameras([(400, 300)], seed=72)[0]
     target = scene.Rng(73).uniform(400 * 300 * 3).reshape(300, 400, 3).astype(np.float32)
-    hip_ctx.set_raster(tile, batch, tile != 32)
+    hip_ctx.set_raster(tile, batch, persistent)
     ds = DeviceSplats(hip_ctx, sc)
     img = render_forward(hip_ctx, ds, [cam])[0]
     r = orc.Render(sc, cam, tile_width=tile)
@@ -202,7 +204,11 @@ def test_raster_configs_vs_oracle(hip_ctx, orc, tile, batch):
     _, v = l1_loss_grad(hip_ctx, img, torch.from_numpy(target).cuda())
     vt, vs, vo, _ = render_backward(hip_ctx, ds, [v])
     ref = r.backward(orc.l1_loss_grad(r.image(), target)[1], 0)
-    errs = [rel_l2(vt, ref["v_transforms"]), rel_l2(vs, ref["v_sh"]), rel_l2(vo, ref["v_raw_opac"])]
-    say(f"  tile {tile} batch {batch}: image err {err:.2e}, grads " + " ".join(f"{e:.1e}" for e in errs))
+    rt = ref["v_transforms"]
+    errs = dict(means=rel_l2(vt[:, :3], rt[:, :3]), quats=rel_l2(vt[:, 3:7], rt[:, 3:7]),
+                log_scales=rel_l2(vt[:, 7:], rt[:, 7:]), opacity=rel_l2(vo, ref["v_raw_opac"]),
+                sh=rel_l2(vs, ref["v_sh"]))
+    say(f"  tile {tile} batch {batch} persistent {persistent}: image err {err:.2e}, grads "
+        + " ".join(f"{k}={e:.1e}" for k, e in errs.items()))
     hip_ctx.set_raster(16, 256, True)
-    assert err <= 1e-4 and max(errs) <= 1e-3
+    assert err <= 1e-4 and max(errs.values()) <= 1e-3
