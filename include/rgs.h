@@ -192,6 +192,20 @@ int rgs_sort_pairs_u32(rgs_ctx* ctx, const uint32_t* keys, const uint32_t* value
                        uint32_t* out_values, int64_t n);
 int rgs_inclusive_scan_u32(rgs_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n);
 
+/* rgs_sort_pairs_u32 without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP stream capture, allocation-free serving loops).  Same results as
+ * rgs_sort_pairs_u32 (stable ascending sort of (key, value) pairs, device
+ * pointers; the outputs may alias the inputs) for n <= RGS_SORT_NOALLOC_MAX_PAIRS;
+ * larger n is rejected with RGS_ERR_INVALID.  Its scratch memory is one
+ * fixed device array per GPU (18.9 MB, held until the process ends), shared
+ * by every call on that GPU: calls that overlap on one GPU (several
+ * contexts or threads) race.  Asynchronous on the context stream; calls no
+ * hipMalloc or hipFree. */
+#define RGS_SORT_NOALLOC_MAX_PAIRS 2097152
+int rgs_sort_pairs_u32_noalloc(rgs_ctx* ctx, const uint32_t* keys, const uint32_t* values, uint32_t* out_keys,
+                               uint32_t* out_values, int64_t n);
+
 /* ---- intermediate state of the last forward, for parity checks ------------
  * Copied into host or device memory; synchronous.  `view` selects a view of
  * the last batch; returns the element count through *count (call with

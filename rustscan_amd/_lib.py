@@ -20,6 +20,8 @@ RGS_ERR_UNSUPPORTED = 5
 
 RGS_MODE_PER_VIEW = 0
 RGS_MODE_BATCHED = 1
+RGS_SORT_NOALLOC_MAX_PAIRS = 2097152
+
 RGS_GRAPH_OFF = 0
 RGS_GRAPH_ON = 1
 
@@ -119,6 +121,7 @@ _SIGNATURES = {
     "rgs_skipped_steps": (c_int64, [c_void_p]),
     "rgs_sort_pairs_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
     "rgs_inclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
+    "rgs_sort_pairs_u32_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
     "rgs_debug_get": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(c_int64)]),
     "rgs_set_debug": (c_int, [c_void_p, c_int]),
     "rgs_get_fragment_counts": (c_int, [c_void_p, POINTER(c_int64), c_int]),

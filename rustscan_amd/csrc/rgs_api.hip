@@ -57,6 +57,9 @@ void launch_l1(hipStream_t s, const float* pred, const float* tgt, uint32_t np, 
 void launch_aos_to_soa(hipStream_t s, const float* aos, int width, uint32_t n, float* soa, uint32_t npad, int plane0);
 void launch_soa_to_aos(hipStream_t s, const float* soa, uint32_t npad, int plane0, int width, uint32_t n, float* aos);
 void launch_fill_f32(hipStream_t s, float* p, float v, size_t n);
+hipError_t noalloc_setup(int device);
+int noalloc_sort_pairs(int device, hipStream_t stream, const uint32_t* keys, const uint32_t* values,
+                       uint32_t* out_keys, uint32_t* out_values, uint32_t n, const char** err);
 }  // namespace rgs
 
 using namespace rgs;
@@ -672,6 +675,8 @@ int rgs_ctx_create(int device, rgs_ctx** out) {
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return RGS_ERR_HIP;
   if (device < 0 || device >= count) return RGS_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return RGS_ERR_HIP;
+  // this device's copy of the no-allocation sort's scratch array
+  if (noalloc_setup(device) != hipSuccess) return RGS_ERR_HIP;
   rgs_ctx* c = new rgs_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1224,6 +1229,20 @@ int rgs_sort_pairs_u32(rgs_ctx* c, const uint32_t* keys, const uint32_t* values,
   release(v2);
   release(scratch);
   if (e != hipSuccess) return fail(c, RGS_ERR_HIP, "sort: %s", hipGetErrorString(e));
+  return RGS_OK;
+}
+
+int rgs_sort_pairs_u32_noalloc(rgs_ctx* c, const uint32_t* keys, const uint32_t* values, uint32_t* ok, uint32_t* ov,
+                               int64_t n) {
+  if (!c || n < 0 || (n > 0 && (!keys || !values || !ok || !ov))) return RGS_ERR_INVALID;
+  if (n > RGS_SORT_NOALLOC_MAX_PAIRS)
+    return fail(c, RGS_ERR_INVALID, "rgs_sort_pairs_u32_noalloc: n = %lld exceeds the maximum of %d pairs",
+                (long long)n, RGS_SORT_NOALLOC_MAX_PAIRS);
+  if (n == 0) return RGS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const char* err = "";
+  const int rc = noalloc_sort_pairs(c->device, c->stream, keys, values, ok, ov, (uint32_t)n, &err);
+  if (rc) return fail(c, rc, "rgs_sort_pairs_u32_noalloc: %s", err);
   return RGS_OK;
 }
 
