@@ -55,12 +55,15 @@ typedef struct rgs_camera {
 /* AdamScaled (engine/optimizer.rs:7-24, 122-162) with the per-column
  * learning-rate scaling tensors of trainer.rs:127-151 folded in:
  * lr_transforms[10] = lr * transform scaling, lr_sh[25] = lr * SH scaling per
- * coefficient (first K used), lr_opacity = lr * opacity scaling. */
+ * coefficient (first K used), lr_opacity = lr * opacity scaling.
+ * weight_decay: AdamScaledConfig::weight_decay (gradient += weight_decay *
+ * parameter before the moments, optimizer.rs:128-130; 0 in RustGS). */
 typedef struct rgs_adam_cfg {
   float beta1, beta2, eps;
   float lr_transforms[10];
   float lr_sh[25];
   float lr_opacity;
+  float weight_decay;
 } rgs_adam_cfg;
 
 enum { RGS_MODE_PER_VIEW = 0, RGS_MODE_BATCHED = 1 };
@@ -129,8 +132,22 @@ int rgs_splats_export_grads(rgs_ctx* ctx, const rgs_splats* s, float* v_transfor
 /* visible[N]: 1.0 where some fragment of the last forward blended the
  * Gaussian, OR over views (rasterize.wgsl:79); synchronous */
 int rgs_splats_export_visible(rgs_ctx* ctx, const rgs_splats* s, float* visible);
-/* AdamScaled::reset (optimizer.rs:62-66): zero moments and step counter */
+/* AdamScaled::reset (optimizer.rs:62-66): zero moments and step counters */
 int rgs_adam_reset(rgs_ctx* ctx, rgs_splats* s);
+/* AdamState::step of the three groups (optimizer.rs:27-32, 46-51):
+ * steps[0] transforms, steps[1] SH coefficients, steps[2] opacities.  Read,
+ * or set (to resume from a checkpoint); synchronous. */
+int rgs_adam_get_steps(rgs_ctx* ctx, const rgs_splats* s, uint32_t* steps);
+int rgs_adam_set_steps(rgs_ctx* ctx, rgs_splats* s, const uint32_t* steps);
+/* AdamState::moment1 (m_*) and moment2 (v_*) of the three groups
+ * (optimizer.rs:27-32) in the reference layouts: [N][10], [N][K][3], [N].
+ * Export: synchronous, any pointer may be NULL.  Import: all six required;
+ * the step counters are left as they are. */
+int rgs_adam_export_moments(rgs_ctx* ctx, const rgs_splats* s, float* m_transforms, float* m_sh, float* m_raw_opac,
+                            float* v_transforms, float* v_sh, float* v_raw_opac);
+int rgs_adam_import_moments(rgs_ctx* ctx, rgs_splats* s, const float* m_transforms, const float* m_sh,
+                            const float* m_raw_opac, const float* v_transforms, const float* v_sh,
+                            const float* v_raw_opac);
 
 /* ---- forward: render_forward (forward/mod.rs:156-294) --------------------
  * Projection + SH (ProjectionBackend, projection.rs:44-50; ProjectVisibleBackend,

@@ -73,7 +73,7 @@ def lib():
         L.orc_l1_loss_grad.restype = c_double
         L.orc_l1_loss_grad.argtypes = [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p]
         L.orc_adam_group.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
-                                     c_float, c_float, ctypes.c_int32]
+                                     c_float, c_float, ctypes.c_int32, c_float]
         L.orc_powi.restype = c_float
         L.orc_powi.argtypes = [c_float, ctypes.c_int32]
         L.orc_forward_loss_f64.restype = c_double
@@ -229,14 +229,14 @@ def l1_loss_grad(pred: np.ndarray, target: np.ndarray, weight: float = 1.0):
     return loss, v
 
 
-def adam_group(param, grad, m, v, lr_row, beta1, beta2, eps, step):
+def adam_group(param, grad, m, v, lr_row, beta1, beta2, eps, step, weight_decay=0.0):
     """In-place Adam on float32 arrays of shape [rows, row_len]."""
     for a in (param, grad, m, v):
         assert a.dtype == np.float32 and a.flags.c_contiguous
     rows, row_len = param.shape
     lr = _f32(lr_row)
     lib().orc_adam_group(rows, row_len, param.ctypes.data, grad.ctypes.data, m.ctypes.data, v.ctypes.data,
-                         lr.ctypes.data, float(beta1), float(beta2), float(eps), int(step))
+                         lr.ctypes.data, float(beta1), float(beta2), float(eps), int(step), float(weight_decay))
 
 
 def powi(base: float, e: int) -> float:

@@ -1220,8 +1220,8 @@ float orc_powi(float a, int32_t b) {
 }
 
 void orc_adam_group(int64_t n_rows, int64_t row_len, float* param, const float* grad, float* m, float* v,
-                    const float* lr_row, float beta1, float beta2, float eps, int32_t step) {
-  // engine/optimizer.rs:122-162
+                    const float* lr_row, float beta1, float beta2, float eps, int32_t step, float weight_decay) {
+  // engine/optimizer.rs:122-162 (weight decay: :128-130)
   const float omb1 = 1.0f - beta1, omb2 = 1.0f - beta2;
   const float bc1 = 1.0f - orc_powi(beta1, step);
   const float bc2 = 1.0f - orc_powi(beta2, step);
@@ -1229,7 +1229,8 @@ void orc_adam_group(int64_t n_rows, int64_t row_len, float* param, const float* 
   for (int64_t r = 0; r < n_rows; ++r)
     for (int64_t c = 0; c < row_len; ++c) {
       const int64_t i = r * row_len + c;
-      const float g = grad[i];
+      float g = grad[i];
+      if (weight_decay != 0.0f) g = g + param[i] * weight_decay;
       const float gsq = g * g;
       const float m1 = m[i] * beta1 + g * omb1;
       const float m2 = v[i] * beta2 + gsq * omb2;

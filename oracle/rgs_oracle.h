@@ -88,7 +88,8 @@ double orc_l1_loss_grad(const float* pred_rgba, const float* target_rgb,
  * AFTER increment.  beta1/beta2/eps as in AdamScaledConfig. */
 void orc_adam_group(int64_t n_rows, int64_t row_len, float* param,
                     const float* grad, float* m, float* v, const float* lr_row,
-                    float beta1, float beta2, float eps, int32_t step);
+                    float beta1, float beta2, float eps, int32_t step,
+                    float weight_decay);
 float orc_powi(float base, int32_t exp);
 
 /* ---- FP64 forward for finite differences ---------------------------------

@@ -59,6 +59,7 @@ class RgsAdamCfg(ctypes.Structure):
         ("lr_transforms", c_float * 10),
         ("lr_sh", c_float * 25),
         ("lr_opacity", c_float),
+        ("weight_decay", c_float),
     ]
 
 
@@ -107,6 +108,10 @@ _SIGNATURES = {
     "rgs_splats_export_grads": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "rgs_splats_export_visible": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_reset": (c_int, [c_void_p, c_void_p]),
+    "rgs_adam_get_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rgs_adam_set_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rgs_adam_export_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),
+    "rgs_adam_import_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),
     "rgs_render_forward": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
                                    POINTER(c_void_p)]),
     "rgs_render_backward": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),

@@ -17,9 +17,10 @@ __host__ __device__ inline float powi_f32(float a, int b) {
   return recip ? 1.0f / r : r;
 }
 
-// Advances the Adam step counter (all three parameter groups of AdamScaled
-// advance together, optimizer.rs:144) and computes the f32 bias corrections,
-// unless this step was invalidated (capacity overflow or non-finite loss).
+// Advances the Adam step counters of the three parameter groups of
+// AdamScaled (transforms, SH, opacity: AdamState::step, optimizer.rs:27-32,
+// 144) and computes each group's f32 bias corrections, unless this step was
+// invalidated (capacity overflow or non-finite loss).
 __global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restrict__ fscal,
                                 uint32_t* __restrict__ step_ctr, float b1, float b2, int nviews, int check_loss) {
   if (threadIdx.x != 0) return;
@@ -30,10 +31,12 @@ __global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restri
     }
   }
   if (counters[kCntStepValid] == 0u) return;
-  const uint32_t step = *step_ctr + 1u;
-  *step_ctr = step;
-  fscal[kFBc1] = 1.0f - powi_f32(b1, (int)step);
-  fscal[kFBc2] = 1.0f - powi_f32(b2, (int)step);
+  for (int g = 0; g < kAdamGroups; ++g) {
+    const uint32_t step = step_ctr[g] + 1u;  // saturating_add in the reference; 2^32 steps are not reached
+    step_ctr[g] = step;
+    fscal[kFBc1 + 2 * g] = 1.0f - powi_f32(b1, (int)step);
+    fscal[kFBc1 + 2 * g + 1] = 1.0f - powi_f32(b2, (int)step);
+  }
 }
 
 // engine/optimizer.rs:122-162 for all groups at once: SoA planes, one
@@ -43,7 +46,8 @@ __global__ void k_adam_prologue(uint32_t* __restrict__ counters, float* __restri
 // are npad = multiple of 64 floats long, so every row is 16-byte aligned).
 // Columns past n are written back unchanged.
 __device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, float b1, float b2, float omb1,
-                                         float omb2, float bc1, float bc2, float eps, float lr) {
+                                         float omb2, float bc1, float bc2, float eps, float lr, float wd) {
+  if (wd != 0.0f) g = g + p * wd;  // optimizer.rs:128-130
   const float gsq = g * g;
   const float m1 = m * b1 + g * omb1;
   const float m2 = v * b2 + gsq * omb2;
@@ -58,24 +62,25 @@ __device__ __forceinline__ void adam_one(float& p, float g, float& m, float& v, 
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, const float* __restrict__ G,
                                               float* __restrict__ M, float* __restrict__ Vv, uint32_t npad,
                                               uint32_t n, const float* __restrict__ lr_planes, float b1, float b2,
-                                              float eps, const float* __restrict__ fscal,
+                                              float eps, float wd, const float* __restrict__ fscal,
                                               const uint32_t* __restrict__ counters) {
   if (counters[kCntStepValid] == 0u) return;
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;  // float4 column
   const uint32_t c0 = 4 * q;
   if (c0 >= n) return;
   const float omb1 = 1.0f - b1, omb2 = 1.0f - b2;
-  const float bc1 = fscal[kFBc1], bc2 = fscal[kFBc2];
+  const int group = adam_group_of_plane((int)blockIdx.y);
+  const float bc1 = fscal[kFBc1 + 2 * group], bc2 = fscal[kFBc1 + 2 * group + 1];
   const float lr = lr_planes[blockIdx.y];
   const size_t i4 = ((size_t)blockIdx.y * npad) / 4 + q;
   float4 p = reinterpret_cast<float4*>(P)[i4];
   const float4 g = reinterpret_cast<const float4*>(G)[i4];
   float4 m = reinterpret_cast<float4*>(M)[i4];
   float4 v = reinterpret_cast<float4*>(Vv)[i4];
-  adam_one(p.x, g.x, m.x, v.x, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
-  if (c0 + 1 < n) adam_one(p.y, g.y, m.y, v.y, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
-  if (c0 + 2 < n) adam_one(p.z, g.z, m.z, v.z, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
-  if (c0 + 3 < n) adam_one(p.w, g.w, m.w, v.w, b1, b2, omb1, omb2, bc1, bc2, eps, lr);
+  adam_one(p.x, g.x, m.x, v.x, b1, b2, omb1, omb2, bc1, bc2, eps, lr, wd);
+  if (c0 + 1 < n) adam_one(p.y, g.y, m.y, v.y, b1, b2, omb1, omb2, bc1, bc2, eps, lr, wd);
+  if (c0 + 2 < n) adam_one(p.z, g.z, m.z, v.z, b1, b2, omb1, omb2, bc1, bc2, eps, lr, wd);
+  if (c0 + 3 < n) adam_one(p.w, g.w, m.w, v.w, b1, b2, omb1, omb2, bc1, bc2, eps, lr, wd);
   reinterpret_cast<float4*>(P)[i4] = p;
   reinterpret_cast<float4*>(M)[i4] = m;
   reinterpret_cast<float4*>(Vv)[i4] = v;
@@ -136,12 +141,12 @@ static uint32_t blocks_for(size_t n, uint32_t cap = 16384) {
 }
 
 void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, uint32_t npad, uint32_t n, int nplanes,
-                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
+                 const float* lr_planes, float b1, float b2, float eps, float wd, float* fscal, uint32_t* counters,
                  uint32_t* step, int nviews, int check_loss) {
   k_adam_prologue<<<1, 64, 0, s>>>(counters, fscal, step, b1, b2, nviews, check_loss);
   if (n == 0 || nplanes <= 0) return;
   const dim3 grid((npad / 4 + 255) / 256, (uint32_t)nplanes);
-  k_adam<<<grid, 256, 0, s>>>(P, G, M, V, npad, n, lr_planes, b1, b2, eps, fscal, counters);
+  k_adam<<<grid, 256, 0, s>>>(P, G, M, V, npad, n, lr_planes, b1, b2, eps, wd, fscal, counters);
 }
 
 void launch_l1(hipStream_t s, const float* pred, const float* tgt, uint32_t np, float scale, float* v_out,

@@ -50,7 +50,7 @@ int launch_raster_bwd(hipStream_t s, int tw, int batch, const RasterArgs& A, uin
 int raster_persistent_grid(int tw, int batch, bool bwd, int num_cus);
 uint32_t raster_items(int tw, uint32_t tiles);
 void launch_adam(hipStream_t s, float* P, const float* G, float* M, float* V, uint32_t npad, uint32_t n, int nplanes,
-                 const float* lr_planes, float b1, float b2, float eps, float* fscal, uint32_t* counters,
+                 const float* lr_planes, float b1, float b2, float eps, float wd, float* fscal, uint32_t* counters,
                  uint32_t* step, int nviews, int check_loss);
 void launch_l1(hipStream_t s, const float* pred, const float* tgt, uint32_t np, float scale, float* v_out,
                float* loss_sum);
@@ -85,7 +85,7 @@ struct rgs_splats {
   uint32_t npad = 0;
   int deg = 0, nk = 1, nplanes = 11;
   DevBuf P, G, M, V, stats, visible, lr;
-  DevBuf step;  // Adam step counter of this store (AdamState::step, optimizer.rs:27-32)
+  DevBuf step;  // Adam step counters of the three groups (AdamState::step, optimizer.rs:27-32)
 };
 
 struct Workspace {
@@ -527,7 +527,8 @@ void set_lr(rgs_splats* s, const rgs_adam_cfg* cfg, float* host_planes) {
 int enqueue_adam(rgs_ctx* c, rgs_splats* s, const rgs_adam_cfg* cfg, int nviews, int check_loss) {
   StageTimer t(c, RGS_STAGE_ADAM);
   launch_adam(c->stream, s->P.as<float>(), s->G.as<float>(), s->M.as<float>(), s->V.as<float>(), s->npad,
-              (uint32_t)s->n, s->nplanes, s->lr.as<float>(), cfg->beta1, cfg->beta2, cfg->eps, c->ws.fscal.as<float>(),
+              (uint32_t)s->n, s->nplanes, s->lr.as<float>(), cfg->beta1, cfg->beta2, cfg->eps, cfg->weight_decay,
+              c->ws.fscal.as<float>(),
               c->ws.counters.as<uint32_t>(), s->step.as<uint32_t>(), nviews, check_loss);
   return RGS_OK;
 }
@@ -779,7 +780,7 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   rc |= ensure(c, s->stats, (size_t)s->npad * 4 * 7);
   rc |= ensure(c, s->visible, (size_t)s->npad * 4);
   rc |= ensure(c, s->lr, 4 * (size_t)s->nplanes);
-  rc |= ensure(c, s->step, 4);
+  rc |= ensure(c, s->step, 4 * kAdamGroups);
   if (rc) {
     rgs_splats_destroy(c, s);
     return RGS_ERR_HIP;
@@ -792,7 +793,7 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   launch_fill_f32(c->stream, s->stats.as<float>(), 0.0f, (size_t)s->npad * 7);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
   launch_fill_f32(c->stream, s->lr.as<float>(), 0.0f, s->nplanes);
-  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, 1);
+  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, kAdamGroups);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rgs_splats_destroy(c, s);
     return fail(c, RGS_ERR_HIP, "stream error");
@@ -823,7 +824,21 @@ int rgs_adam_reset(rgs_ctx* c, rgs_splats* s) {
   const size_t nf = (size_t)s->npad * s->nplanes;
   launch_fill_f32(c->stream, s->M.as<float>(), 0.0f, nf);
   launch_fill_f32(c->stream, s->V.as<float>(), 0.0f, nf);
-  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, 1);
+  launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, kAdamGroups);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RGS_OK;
+}
+
+int rgs_adam_get_steps(rgs_ctx* c, const rgs_splats* s, uint32_t* steps) {
+  if (!c || !s || !steps) return RGS_ERR_INVALID;
+  HIPCHK(c, hipMemcpyAsync(steps, s->step.p, 4 * kAdamGroups, hipMemcpyDefault, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RGS_OK;
+}
+
+int rgs_adam_set_steps(rgs_ctx* c, rgs_splats* s, const uint32_t* steps) {
+  if (!c || !s || !steps) return RGS_ERR_INVALID;
+  HIPCHK(c, hipMemcpyAsync(s->step.p, steps, 4 * kAdamGroups, hipMemcpyDefault, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RGS_OK;
 }
@@ -884,6 +899,51 @@ int rgs_splats_export_grads(rgs_ctx* c, const rgs_splats* s, float* vt, float* v
   if (!rc) rc = get(s->G.as<float>(), vop, 1, 10);
   if (!rc) rc = get(s->G.as<float>(), vsh, 3 * s->nk, 11);
   if (!rc) rc = get(s->stats.as<float>(), stats, 7, 0);
+  release(tmp);
+  return rc;
+}
+
+int rgs_adam_export_moments(rgs_ctx* c, const rgs_splats* s, float* m_tf, float* m_sh, float* m_op, float* v_tf,
+                            float* v_sh, float* v_op) {
+  if (!c || !s) return RGS_ERR_INVALID;
+  const uint32_t n = (uint32_t)s->n;
+  DevBuf tmp;
+  if (ensure(c, tmp, (size_t)n * std::max(10, 3 * s->nk) * 4)) return RGS_ERR_HIP;
+  auto get = [&](const float* planes, float* dst, int width, int plane0) -> int {
+    if (!dst) return RGS_OK;
+    launch_soa_to_aos(c->stream, planes, s->npad, plane0, width, n, tmp.as<float>());
+    HIPCHK(c, hipMemcpyAsync(dst, tmp.p, (size_t)n * width * 4, hipMemcpyDefault, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RGS_OK;
+  };
+  int rc = get(s->M.as<float>(), m_tf, 10, 0);
+  if (!rc) rc = get(s->M.as<float>(), m_op, 1, 10);
+  if (!rc) rc = get(s->M.as<float>(), m_sh, 3 * s->nk, 11);
+  if (!rc) rc = get(s->V.as<float>(), v_tf, 10, 0);
+  if (!rc) rc = get(s->V.as<float>(), v_op, 1, 10);
+  if (!rc) rc = get(s->V.as<float>(), v_sh, 3 * s->nk, 11);
+  release(tmp);
+  return rc;
+}
+
+int rgs_adam_import_moments(rgs_ctx* c, rgs_splats* s, const float* m_tf, const float* m_sh, const float* m_op,
+                            const float* v_tf, const float* v_sh, const float* v_op) {
+  if (!c || !s || !m_tf || !m_sh || !m_op || !v_tf || !v_sh || !v_op) return RGS_ERR_INVALID;
+  const uint32_t n = (uint32_t)s->n;
+  DevBuf tmp;
+  if (ensure(c, tmp, (size_t)n * std::max(10, 3 * s->nk) * 4)) return RGS_ERR_HIP;
+  auto put = [&](float* planes, const float* src, int width, int plane0) -> int {
+    HIPCHK(c, hipMemcpyAsync(tmp.p, src, (size_t)n * width * 4, hipMemcpyDefault, c->stream));
+    launch_aos_to_soa(c->stream, tmp.as<float>(), width, n, planes, s->npad, plane0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RGS_OK;
+  };
+  int rc = put(s->M.as<float>(), m_tf, 10, 0);
+  if (!rc) rc = put(s->M.as<float>(), m_op, 1, 10);
+  if (!rc) rc = put(s->M.as<float>(), m_sh, 3 * s->nk, 11);
+  if (!rc) rc = put(s->V.as<float>(), v_tf, 10, 0);
+  if (!rc) rc = put(s->V.as<float>(), v_op, 1, 10);
+  if (!rc) rc = put(s->V.as<float>(), v_sh, 3 * s->nk, 11);
   release(tmp);
   return rc;
 }
@@ -1117,9 +1177,20 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
   HIPCHK(c, hipMemcpyAsync(sp.p, s->P.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(sm.p, s->M.p, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(sv.p, s->V.p, bytes, hipMemcpyDeviceToDevice, c->stream));
-  uint32_t saved_step = 0;
-  HIPCHK(c, hipMemcpyAsync(&saved_step, s->step.p, 4, hipMemcpyDeviceToHost, c->stream));
+  uint32_t saved_step[kAdamGroups] = {};
+  HIPCHK(c, hipMemcpyAsync(saved_step, s->step.p, 4 * kAdamGroups, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int64_t saved_skipped = rgs_skipped_steps(c);
+  // parameters, moments and step counter as the caller left them: every
+  // candidate is timed from this state, and it is what the caller gets back
+  auto restore = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(s->P.p, sp.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->M.p, sm.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->V.p, sv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->step.p, saved_step, 4 * kAdamGroups, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RGS_OK;
+  };
   const rgs_raster_cfg orig = c->raster;
   rgs_raster_cfg best = orig;
   float best_ms = 1e30f;
@@ -1131,6 +1202,7 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
     for (uint32_t b : bs) {
       rgs_raster_cfg rc_cfg{tw, b, 1};
       if (rgs_set_raster_config(c, &rc_cfg)) continue;
+      if ((rc = restore())) return rc;
       int r = rgs_train_step(c, s, cams, targets, nv, cfg, nullptr);  // warm-up (sizes, captures)
       if (r) continue;
       hipEventRecord(e0, c->stream);
@@ -1151,10 +1223,18 @@ int rgs_autotune(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const float*
     }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  HIPCHK(c, hipMemcpyAsync(s->P.p, sp.p, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(s->M.p, sm.p, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(s->V.p, sv.p, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(s->step.p, &saved_step, 4, hipMemcpyHostToDevice, c->stream));
+  if (c->step_pending) {
+    HIPCHK(c, hipEventSynchronize(c->step_done));
+    c->step_pending = false;
+  }
+  if ((rc = restore())) return rc;
+  // the tuning steps count in neither the skipped steps nor the overflow
+  // state the next step reacts to (each candidate sized its own plan)
+  if (c->ws.counters.p) {
+    if (saved_skipped >= 0)
+      launch_set_u32(c->stream, c->ws.counters.as<uint32_t>() + kCntSkipped, (uint32_t)saved_skipped);
+    clear_overflow_sticky(c);
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   release(sp);
   release(sm);

@@ -63,11 +63,16 @@ enum Counter : int {
 
 // Float scalars block
 enum FScalar : int {
-  kFLossBase = 0,  // [kMaxViews] per-view loss sums
-  kFBc1 = kMaxViews,
-  kFBc2,
+  kFLossBase = 0,      // [kMaxViews] per-view loss sums
+  kFBc1 = kMaxViews,   // Adam bias corrections (1 - b1^t, 1 - b2^t) of each group g at kFBc1 + 2g, + 2g + 1
   kFNum = kMaxViews + 8,
 };
+
+// Adam parameter groups of AdamScaled (optimizer.rs:46-51), each with its own
+// step counter: 0 = transforms (planes 0-9), 1 = SH coefficients (planes 11
+// and up), 2 = opacity logits (plane 10).
+constexpr int kAdamGroups = 3;
+__host__ __device__ inline int adam_group_of_plane(int plane) { return plane < 10 ? 0 : (plane == 10 ? 2 : 1); }
 
 constexpr int kAcc = 16;
 

@@ -174,6 +174,18 @@ class DeviceSplats:
                                                             vo.ctypes.data, st.ctypes.data))
         return vt, vs, vo, st
 
+    def adam_moments(self):
+        """(moment1, moment2) of AdamState, each (transforms [N,10], sh [N,K,3], raw_opac [N])."""
+        arrs = [np.zeros((self.n, 10), np.float32), np.zeros((self.n, self.k, 3), np.float32),
+                np.zeros(self.n, np.float32), np.zeros((self.n, 10), np.float32),
+                np.zeros((self.n, self.k, 3), np.float32), np.zeros(self.n, np.float32)]
+        self.ctx.check(self.ctx.lib.rgs_adam_export_moments(self.ctx.h, self.h, *[a.ctypes.data for a in arrs]))
+        return tuple(arrs[:3]), tuple(arrs[3:])
+
+    def set_adam_moments(self, m, v):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (*m, *v)]
+        self.ctx.check(self.ctx.lib.rgs_adam_import_moments(self.ctx.h, self.h, *[a.ctypes.data for a in arrs]))
+
     def visible(self) -> np.ndarray:
         out = np.zeros(self.n, np.float32)
         self.ctx.check(self.ctx.lib.rgs_splats_export_visible(self.ctx.h, self.h, out.ctypes.data))
@@ -249,8 +261,6 @@ class AdamScaled:
         self.opacity_scaling = float(s)
 
     def to_cfg(self) -> L.RgsAdamCfg:
-        if self.config.weight_decay != 0.0:
-            raise ValueError("weight decay is not supported by the fused Adam (the trainer uses 0)")
         c = L.RgsAdamCfg()
         c.beta1, c.beta2, c.eps = float(np.float32(self.config.betas[0])), float(np.float32(self.config.betas[1])), \
             float(np.float32(self.config.eps))
@@ -260,6 +270,7 @@ class AdamScaled:
         for i in range(25):
             c.lr_sh[i] = float(np.float32(self.sh_scaling[i]) * lr)
         c.lr_opacity = float(np.float32(self.opacity_scaling) * lr)
+        c.weight_decay = float(np.float32(self.config.weight_decay))
         return c
 
     def step_device_splats(self, ctx: Context, splats: DeviceSplats):
@@ -268,6 +279,18 @@ class AdamScaled:
 
     def reset(self, ctx: Context, splats: DeviceSplats):
         ctx.check(ctx.lib.rgs_adam_reset(ctx.h, splats.h))
+
+    @staticmethod
+    def steps(ctx: Context, splats: DeviceSplats) -> list:
+        """AdamState::step of the transforms, SH and opacity groups."""
+        out = (ctypes.c_uint32 * 3)()
+        ctx.check(ctx.lib.rgs_adam_get_steps(ctx.h, splats.h, out))
+        return list(out)
+
+    @staticmethod
+    def set_steps(ctx: Context, splats: DeviceSplats, steps):
+        arr = (ctypes.c_uint32 * 3)(*[int(x) for x in steps])
+        ctx.check(ctx.lib.rgs_adam_set_steps(ctx.h, splats.h, arr))
 
 
 def lr_at(initial: float, final_value: float, iteration: int, iterations: int, decay_iterations=None) -> float:

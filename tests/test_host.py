@@ -109,7 +109,7 @@ def test_optimizer_configuration_columns():
     assert c.lr_opacity == f(5e-2)
     assert list(c.lr_sh)[:16] == [f(2.5e-3)] + [f(1.25e-4)] * 15
     assert (c.beta1, c.beta2, c.eps) == (f(0.9), f(0.999), f(1e-8))
-    bad = AdamScaled()
-    bad.config.weight_decay = 0.1
-    with pytest.raises(ValueError):
-        bad.to_cfg()
+    assert c.weight_decay == 0.0
+    wd = AdamScaled()
+    wd.config.weight_decay = 0.1
+    assert wd.to_cfg().weight_decay == f(0.1)

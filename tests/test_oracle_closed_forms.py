@@ -207,6 +207,18 @@ def test_adam_closed_form(orc):
     v2 = f2 * o2 * g1d ** 2 + o2 * g2d ** 2
     upd = (m2 / bc1) / (np.sqrt(v2 / bc2) + 1e-8)
     np.testing.assert_allclose(p, p1 - lr * upd, rtol=2e-6, atol=1e-9)
+    # weight decay (optimizer.rs:128-130): the gradient becomes g + wd * p
+    # before the moments, so the first step moves p by lr * g' / (|g'| + eps)
+    wd = np.float32(0.5)
+    q = np.array([[1.0, -2.0, 4.0]], np.float32)
+    gq = np.array([[0.25, 3.0, -2.0]], np.float32)
+    mq, vq = np.zeros_like(q), np.zeros_like(q)
+    q0 = q.copy()
+    orc.adam_group(q, gq.copy(), mq, vq, lr, b1, b2, eps, 1, weight_decay=wd)
+    gd = gq.astype(np.float64) + 0.5 * q0.astype(np.float64)  # (0.75, 2.0, 0.0)
+    np.testing.assert_allclose(q, q0 - lr * gd / (np.abs(gd) + 1e-8), rtol=1e-6, atol=1e-9)
+    assert q[0, 2] == q0[0, 2]  # g' = 0 there: no move
+    np.testing.assert_allclose(mq, (1 - 0.9) * gd, rtol=1e-6)
     # the f32 powi of the bias correction
     # binary exponentiation: repeated squaring doubles the relative error of
     # each power, so the bound grows like step * 2^-24 (and is 0 at step 1)
