@@ -8,7 +8,8 @@ of radius 5 around the scene, targets rendered from a second synthetic scene
 (seed 6).  One step = forward of the 8 views, L1 loss (weight 1/8 per view),
 backward summed over the views, one fused Adam update (constant learning
 rates: means 1.6e-4, scales 5e-3, quats 1e-3, opacity 5e-2, SH DC 2.5e-3, SH
-rest 1.25e-4).
+rest 1.25e-4), and the densification statistics accumulated per view as
+the reference's train_step does.
 
 `value` is batched steps/s: all views in one launch per stage, the step
 captured as a HIP graph, tile width and batch depth from the autotuner.  The
@@ -33,6 +34,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# the reference's train_step accumulates the densification statistics every
+# step (trainer.rs:366-373); the timed steps do the same
+from rustscan_amd._lib import RGS_DENSIFY_ACCUMULATE  # noqa: E402
 
 METRIC = "train steps/s @1M Gaussians 720p: batched vs per-view, grad rel err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
@@ -93,7 +98,7 @@ def run_config3(ctx, opt, init, gt, steps: int = 200) -> dict:
         targets.append(render_forward(ctx, g, [c])[0][..., :3].contiguous())
     g.close()
     splats = DeviceSplats(ctx, init)
-    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0)
+    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0, densify=RGS_DENSIFY_ACCUMULATE)
     first = [trainer.train_step(splats, [cams[i]], [targets[i]], batched=True, graph=True, read_loss=True)[0]
              for i in range(8)]
     ctx.sync()
@@ -135,7 +140,7 @@ def run_config5(args, ctx, opt) -> dict:
     gen = torch.Generator(device="cuda").manual_seed(10)
     targets = [torch.rand((c.height, c.width, 3), device="cuda", generator=gen) for c in cams]
     splats = DeviceSplats(ctx, sc)
-    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams))
+    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams), densify=RGS_DENSIFY_ACCUMULATE)
     log(f"config5 setup {time.perf_counter() - t0:.1f}s")
     trainer.train_step(splats, cams, targets, batched=True, graph=True)
     ctx.sync()
@@ -210,7 +215,7 @@ def main() -> int:
     torch.cuda.synchronize()
     opt = training_optimizer(3, lr_position=1.6e-4, lr_rotation=1e-3, lr_scale=5e-3, lr_opacity=5e-2,
                              lr_sh_dc=2.5e-3, lr_sh_rest=1.25e-4)
-    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams))
+    trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams), densify=RGS_DENSIFY_ACCUMULATE)
     splats = DeviceSplats(ctx, init)
     log(f"setup {time.perf_counter() - t_setup:.1f}s")
 

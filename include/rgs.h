@@ -86,7 +86,18 @@ typedef struct rgs_step_cfg {
   int check_loss;    /* skip the update when the loss is not finite
                         (trainer.rs:297-302) */
   rgs_adam_cfg adam;
+  int densify;       /* RGS_DENSIFY_* flags: accumulate the densification
+                        statistics of each view (accumulate_gradients,
+                        trainer.rs:651-728; see rgs_densify_export) */
 } rgs_step_cfg;
+
+enum {
+  RGS_DENSIFY_ACCUMULATE = 1,        /* update the accumulators each step (per view) */
+  RGS_DENSIFY_ACTUAL_VISIBILITY = 2, /* visible_observations += actual visibility
+                                        (use_actual_visibility), else += 1 */
+  RGS_DENSIFY_COLLECT_VISIBILITY = 4 /* actual_visible_observations += actual visibility
+                                        (collect_actual_visibility_diagnostics) */
+};
 
 typedef struct rgs_raster_cfg {
   uint32_t tile_width;  /* 8, 16 or 32 (reference: 16, forward/mod.rs:25) */
@@ -134,6 +145,18 @@ int rgs_splats_export_grads(rgs_ctx* ctx, const rgs_splats* s, float* v_transfor
 int rgs_splats_export_visible(rgs_ctx* ctx, const rgs_splats* s, float* visible);
 /* AdamScaled::reset (optimizer.rs:62-66): zero moments and step counters */
 int rgs_adam_reset(rgs_ctx* ctx, rgs_splats* s);
+/* Densification accumulators of the trainer (trainer.rs:66-75), updated by
+ * rgs_train_step when cfg->densify has RGS_DENSIFY_ACCUMULATE, once per view
+ * (a one-view step is the reference's accumulate_gradients, trainer.rs:651-728;
+ * a step over several views accumulates as that many one-view steps at the
+ * same parameters); skipped steps add nothing.  out: [10][N], the
+ * accumulators in this order: grad_2d (mean |v_mean| over xyz), screen_grad_2d
+ * (|screen v_xy|), abs_grad_2d (|sum |v_xy||), abs_pixel_grad_2d,
+ * pixel_coverage, camera_depth, grad_color (mean |v_sh|), num_observations,
+ * visible_observations, actual_visible_observations.  Synchronous.
+ * rgs_densify_reset zeroes them (reset_accumulators, trainer.rs:861-871). */
+int rgs_densify_export(rgs_ctx* ctx, const rgs_splats* s, float* out);
+int rgs_densify_reset(rgs_ctx* ctx, rgs_splats* s);
 /* AdamState::step of the three groups (optimizer.rs:27-32, 46-51):
  * steps[0] transforms, steps[1] SH coefficients, steps[2] opacities.  Read,
  * or set (to resume from a checkpoint); synchronous. */

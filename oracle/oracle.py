@@ -348,3 +348,49 @@ def inclusive_scan(x):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+DENSIFY_FIELDS = ["grad_2d", "screen_grad_2d", "abs_grad_2d", "abs_pixel_grad_2d", "pixel_coverage", "camera_depth",
+                  "grad_color", "num_observations", "visible_observations", "actual_visible_observations"]
+
+
+class DensifyAccumulators:
+    """The trainer's densification accumulators and their update, restated
+    from RustGS/src/training/engine/trainer.rs:66-75 (state), :651-728
+    (accumulate_gradients) and :861-871 (reset_accumulators), in float32.
+
+    accumulate() takes one view's gradients: v_transforms [N,10] (the
+    post-projection transform gradient), screen_stats [N,7], v_sh [N,K,3]
+    and the forward's visible [N] (0/1)."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.reset()
+
+    def reset(self):
+        self.acc = {k: np.zeros(self.n, np.float32) for k in DENSIFY_FIELDS}
+
+    def accumulate(self, v_transforms, screen_stats, v_sh, visible, use_actual_visibility=False,
+                   collect_actual_visibility=False):
+        f = np.float32
+        vt = np.asarray(v_transforms, f)
+        st = np.asarray(screen_stats, f)
+        vs = np.asarray(v_sh, f)
+        vis = np.asarray(visible, f)
+        a = self.acc
+        # trainer.rs:663-668 slice(0..3).abs().mean_dim(1)
+        a["grad_2d"] += np.abs(vt[:, 0:3]).mean(axis=1, dtype=f)
+        # :669-675 and :676-682 sqrt(sum of squares) of stats 0..2 and 2..4
+        a["screen_grad_2d"] += np.sqrt((st[:, 0:2] ** 2).sum(axis=1, dtype=f))
+        a["abs_grad_2d"] += np.sqrt((st[:, 2:4] ** 2).sum(axis=1, dtype=f))
+        # :683-694 columns 4, 5, 6 as they are
+        a["abs_pixel_grad_2d"] += st[:, 4]
+        a["pixel_coverage"] += st[:, 5]
+        a["camera_depth"] += st[:, 6]
+        # :695-701 abs().mean_dim(2).mean_dim(1): over channels, then coefficients
+        a["grad_color"] += np.abs(vs).mean(axis=2, dtype=f).mean(axis=1, dtype=f)
+        # :717-727
+        a["num_observations"] += f(1.0)
+        if collect_actual_visibility:
+            a["actual_visible_observations"] += vis
+        a["visible_observations"] += vis if use_actual_visibility else f(1.0)

@@ -22,6 +22,12 @@ RGS_MODE_PER_VIEW = 0
 RGS_MODE_BATCHED = 1
 RGS_SORT_NOALLOC_MAX_PAIRS = 2097152
 
+RGS_DENSIFY_ACCUMULATE = 1
+RGS_DENSIFY_ACTUAL_VISIBILITY = 2
+RGS_DENSIFY_COLLECT_VISIBILITY = 4
+DENSIFY_FIELDS = ["grad_2d", "screen_grad_2d", "abs_grad_2d", "abs_pixel_grad_2d", "pixel_coverage", "camera_depth",
+                  "grad_color", "num_observations", "visible_observations", "actual_visible_observations"]
+
 RGS_GRAPH_OFF = 0
 RGS_GRAPH_ON = 1
 
@@ -72,6 +78,7 @@ class RgsStepCfg(ctypes.Structure):
         ("loss_weight", c_float),
         ("check_loss", c_int),
         ("adam", RgsAdamCfg),
+        ("densify", c_int),
     ]
 
 
@@ -108,6 +115,8 @@ _SIGNATURES = {
     "rgs_splats_export_grads": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "rgs_splats_export_visible": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_reset": (c_int, [c_void_p, c_void_p]),
+    "rgs_densify_export": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rgs_densify_reset": (c_int, [c_void_p, c_void_p]),
     "rgs_adam_get_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_set_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_export_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),

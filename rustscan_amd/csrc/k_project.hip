@@ -348,20 +348,30 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
 }
 
 // ---------------------------------------------------------------- backward
-// SH vector-Jacobian product (helpers.wgsl:289-380): g[k*3+ch] += basis_k * vc[ch]
+// SH vector-Jacobian product (helpers.wgsl:289-380): g[k*3+ch] += basis_k * vc[ch];
+// *abs_basis receives the sum of |basis_k| over k (the colour-gradient
+// statistic of the densification accumulators uses it).
 template <int DEG>
-__device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const float vc[3], float* g) {
+__device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const float vc[3], float* g, float* abs_basis) {
+  float ab = 0.0f;
   auto put = [&](int k, float b) {
+    ab += fabsf(b);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) g[k * 3 + ch] += b * vc[ch];
   };
   put(0, RGS_F(0.28209479177387814));
-  if (DEG == 0) return;
+  if (DEG == 0) {
+    *abs_basis = ab;
+    return;
+  }
   const float fTmp0A = RGS_F(0.48860251190292);
   put(1, -fTmp0A * y);
   put(2, fTmp0A * z);
   put(3, -fTmp0A * x);
-  if (DEG == 1) return;
+  if (DEG == 1) {
+    *abs_basis = ab;
+    return;
+  }
   const float z2 = z * z;
   const float fTmp0B = RGS_F(-1.092548430592079) * z;
   const float fTmp1A = RGS_F(0.5462742152960395);
@@ -372,7 +382,10 @@ __device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const floa
   put(6, RGS_F(0.9461746957575601) * z2 - RGS_F(0.3153915652525201));
   put(7, fTmp0B * x);
   put(8, fTmp1A * fC1);
-  if (DEG == 2) return;
+  if (DEG == 2) {
+    *abs_basis = ab;
+    return;
+  }
   const float fTmp0C = RGS_F(-2.285228997322329) * z2 + RGS_F(0.4570457994644658);
   const float fTmp1B = RGS_F(1.445305721320277) * z;
   const float fTmp2A = RGS_F(-0.5900435899266435);
@@ -386,7 +399,10 @@ __device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const floa
   put(13, fTmp0C * x);
   put(14, fTmp1B * fC1);
   put(15, fTmp2A * fC2);
-  if (DEG == 3) return;
+  if (DEG == 3) {
+    *abs_basis = ab;
+    return;
+  }
   const float fTmp0D = z * (RGS_F(-4.683325804901025) * z2 + RGS_F(2.007139630671868));
   const float fTmp1C = RGS_F(3.31161143515146) * z2 - RGS_F(0.47308734787878);
   const float fTmp2B = RGS_F(-1.770130769779931) * z;
@@ -403,6 +419,7 @@ __device__ __forceinline__ void sh_vjp_acc(float x, float y, float z, const floa
   put(22, fTmp1C * fC1);
   put(23, fTmp2B * fC2);
   put(24, fTmp3A * fC3);
+  *abs_basis = ab;
 }
 
 // Per-instance rasterizer gradient accumulator (written with atomics by the
@@ -445,7 +462,15 @@ struct S3 {
 
 // grads: SoA planes like the parameters (59 planes at SH3).
 // stats: [7][npad] screen-gradient statistics (project_backwards.wgsl:33-36,106-108), summed over views.
-// accumulate != 0 adds into grads/stats instead of overwriting (per-view launches).
+// accumulate != 0 adds into grads/stats/dinc instead of overwriting (per-view launches).
+// dinc (optional): [5][npad] this step's increments of the densification
+// accumulators that are not plain sums of the screen statistics
+// (accumulate_gradients, trainer.rs:651-728, applied once per view):
+//   [0] mean |v_mean| over xyz  [1] |screen v_xy|  [2] |sum |v_xy||
+//   [3] mean |v_sh| over coefficients and channels  [4] views where a
+//   fragment blended the Gaussian (from vis: one byte per instance).
+// They are added to the accumulators by k_densify_commit once the step is
+// known to be valid.
 //
 // Symmetric matrices of the chain (covariances and their gradients) are
 // handled as symmetric: the reference forms them as literal products, whose
@@ -455,7 +480,8 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
                                                      const ViewDev* __restrict__ views, int v0, int nv, float blur,
                                                      float* __restrict__ acc, const uint32_t* __restrict__ keys_by_inst,
                                                      float* __restrict__ grads, float* __restrict__ stats,
-                                                     int accumulate, const uint32_t* __restrict__ counters) {
+                                                     int accumulate, const uint32_t* __restrict__ counters,
+                                                     float* __restrict__ dinc, const uint8_t* __restrict__ vis) {
   constexpr int NK = (DEG + 1) * (DEG + 1);
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= n) return;
@@ -463,6 +489,19 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
   auto out = [&](int plane, float val) {
     float* p = grads + (size_t)plane * npad + gid;
     *p = accumulate ? (*p + val) : val;
+  };
+  // densification increments of this launch's views
+  float d_g2 = 0.f, d_sg = 0.f, d_ag = 0.f, d_gc = 0.f, d_vis = 0.f;
+  if (dinc && vis)
+    for (int v = v0; v < v0 + nv; ++v) d_vis += vis[(size_t)v * n + gid] ? 1.0f : 0.0f;
+  auto out_dinc = [&]() {
+    if (!dinc) return;
+    const float d[5] = {d_g2, d_sg, d_ag, d_gc, d_vis};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      float* p = dinc + (size_t)k * npad + gid;
+      *p = accumulate ? (*p + d[k]) : d[k];
+    }
   };
 
   // Pass 1: the SH colour VJP (FP32: the basis is bounded and its products
@@ -489,9 +528,15 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
       const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
       const float dd = (dx * dx + dy * dy) + dz * dz;
       const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
-      sh_vjp_acc<DEG>(dx * rs, dy * rs, dz * rs, vcol, gsh);
+      float abs_basis;
+      sh_vjp_acc<DEG>(dx * rs, dy * rs, dz * rs, vcol, gsh, &abs_basis);
+      // mean over coefficients and channels of |basis_k * vcol_c|
+      d_gc += (abs_basis * ((fabsf(vcol[0]) + fabsf(vcol[1])) + fabsf(vcol[2]))) / (float)(3 * NK);
     }
-    if (!any && accumulate) return;
+    if (!any && accumulate) {
+      out_dinc();
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < NK * 3; ++i) out(11 + i, gsh[i]);
   }
@@ -534,6 +579,8 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     const float vca = a2.x;
     // screen statistics [0..3]
     st[0] += a0.x; st[1] += a0.y; st[2] += a2.z; st[3] += a2.w;
+    d_sg += rgs_sqrt(a0.x * a0.x + a0.y * a0.y);
+    d_ag += rgs_sqrt(a2.z * a2.z + a2.w * a2.w);
     const float mcz_f = ((V.vm[2] * mean[0] + V.vm[6] * mean[1]) + V.vm[10] * mean[2]) + V.vm[14];
     if (mcz_f < RGS_F(0.01)) continue;
     {
@@ -625,9 +672,15 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     }
     vm2 += -fx * rz2 * vJ00 - fy * rz2 * vJ11;
     // v_mean = Rv^T v_mean_c
+    {
+      float vmv[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-      gmean[r] += (float)((double)V.vm[r * 4] * vm0 + (double)V.vm[r * 4 + 1] * vm1 + (double)V.vm[r * 4 + 2] * vm2);
+      for (int r = 0; r < 3; ++r) {
+        vmv[r] = (float)((double)V.vm[r * 4] * vm0 + (double)V.vm[r * 4 + 1] * vm1 + (double)V.vm[r * 4 + 2] * vm2);
+        gmean[r] += vmv[r];
+      }
+      d_g2 += ((fabsf(vmv[0]) + fabsf(vmv[1])) + fabsf(vmv[2])) / 3.0f;
+    }
     // v_cov_cam = J^T vc J (symmetric)
     S3 vcc;
     vcc.xx = jx * jx * vxx;
@@ -688,6 +741,37 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
       *p = accumulate ? (*p + st[k]) : st[k];
     }
   }
+  out_dinc();
+}
+
+// Adds one step's densification increments to the accumulators
+// (accumulate_gradients, trainer.rs:651-728), skipped when the step was
+// invalidated (capacity overflow, non-finite loss: the reference returns
+// before accumulating, trainer.rs:297-302).  dens: [10][npad]
+//   [0] grad_2d  [1] screen_grad_2d  [2] abs_grad_2d  [3] abs_pixel_grad_2d
+//   [4] pixel_coverage  [5] camera_depth  [6] grad_color  [7] num_observations
+//   [8] visible_observations  [9] actual_visible_observations
+// stats [4..6] are the step's sums over views of coverage * |abs v_xy|,
+// coverage and coverage * depth, which accumulate linearly.
+__global__ __launch_bounds__(256) void k_densify_commit(float* __restrict__ dens, const float* __restrict__ dinc,
+                                                        const float* __restrict__ stats, uint32_t npad, uint32_t n,
+                                                        float nviews, int flags,
+                                                        const uint32_t* __restrict__ counters) {
+  if (counters[kCntStepValid] == 0u) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  auto add = [&](int k, float x) { dens[(size_t)k * npad + g] += x; };
+  const float vis = dinc[4 * (size_t)npad + g];
+  add(0, dinc[g]);
+  add(1, dinc[(size_t)npad + g]);
+  add(2, dinc[2 * (size_t)npad + g]);
+  add(3, stats[4 * (size_t)npad + g]);
+  add(4, stats[5 * (size_t)npad + g]);
+  add(5, stats[6 * (size_t)npad + g]);
+  add(6, dinc[3 * (size_t)npad + g]);
+  add(7, nviews);
+  add(8, (flags & 2) ? vis : nviews);
+  if (flags & 4) add(9, vis);
 }
 
 // ------------------------------------------------------------------ launch
@@ -707,16 +791,22 @@ void launch_project_fwd(int deg, hipStream_t s, const float* P, uint32_t npad, u
 
 void launch_project_bwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
                         int v0, int nv, float blur, float* acc, const uint32_t* keys_by_inst, float* grads,
-                        float* stats, int accumulate, const uint32_t* counters) {
+                        float* stats, int accumulate, const uint32_t* counters, float* dinc, const uint8_t* vis) {
   if (n == 0) return;
   const dim3 grid((n + 255) / 256), block(256);
   switch (deg) {
-    case 0: k_project_bwd<0><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters); break;
-    case 1: k_project_bwd<1><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters); break;
-    case 2: k_project_bwd<2><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters); break;
-    case 3: k_project_bwd<3><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters); break;
-    default: k_project_bwd<4><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters); break;
+    case 0: k_project_bwd<0><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters, dinc, vis); break;
+    case 1: k_project_bwd<1><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters, dinc, vis); break;
+    case 2: k_project_bwd<2><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters, dinc, vis); break;
+    case 3: k_project_bwd<3><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters, dinc, vis); break;
+    default: k_project_bwd<4><<<grid, block, 0, s>>>(P, npad, n, views, v0, nv, blur, acc, keys_by_inst, grads, stats, accumulate, counters, dinc, vis); break;
   }
+}
+
+void launch_densify_commit(hipStream_t s, float* dens, const float* dinc, const float* stats, uint32_t npad,
+                           uint32_t n, int nviews, int flags, const uint32_t* counters) {
+  if (n == 0) return;
+  k_densify_commit<<<(n + 255) / 256, 256, 0, s>>>(dens, dinc, stats, npad, n, (float)nviews, flags, counters);
 }
 
 }  // namespace rgs

@@ -363,6 +363,9 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
       if (A.visible)
         for (uint32_t j = tid; j < cnt; j += NT)
           if (sv[j]) A.visible[sg[j]] = 1.0f;
+      if (A.vis_inst)
+        for (uint32_t j = tid; j < cnt; j += NT)
+          if (sv[j]) A.vis_inst[(size_t)v * A.n + sg[j]] = 1u;
     }
     float l1 = 0.0f;
     if (inside) {

@@ -23,7 +23,9 @@ void launch_project_fwd(int deg, hipStream_t s, const float* P, uint32_t npad, u
                         uint32_t* cnt, uint32_t* counters, int count_ib);
 void launch_project_bwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
                         int v0, int nv, float blur, float* acc, const uint32_t* keys_by_inst, float* grads,
-                        float* stats, int accumulate, const uint32_t* counters);
+                        float* stats, int accumulate, const uint32_t* counters, float* dinc, const uint8_t* vis);
+void launch_densify_commit(hipStream_t s, float* dens, const float* dinc, const float* stats, uint32_t npad,
+                           uint32_t n, int nviews, int flags, const uint32_t* counters);
 uint32_t radix_grid(uint64_t cap);
 uint64_t radix_scratch_words(uint64_t cap);
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
@@ -85,6 +87,9 @@ struct rgs_splats {
   uint32_t npad = 0;
   int deg = 0, nk = 1, nplanes = 11;
   DevBuf P, G, M, V, stats, visible, lr;
+  // densification accumulators [10][npad] (trainer.rs:66-75) and one step's
+  // increments [5][npad] (see k_densify_commit)
+  DevBuf dens, dinc;
   DevBuf step;  // Adam step counters of the three groups (AdamState::step, optimizer.rs:27-32)
 };
 
@@ -95,7 +100,7 @@ struct Workspace {
   size_t cap_isect = 0;  // intersection capacity
   uint32_t tiles_cap = 0;
   DevBuf views, proj, keys, vals, cnt, scnt, sk1, sv1, sk2, sv2, offs, ik0, iv0, ik1, iv1, ranges, acc, acc_dbg, counters,
-      fscal, scratch;
+      fscal, scratch, visi;
   std::vector<DevBuf> own_img;  // images kept internal when the caller passes none
 };
 
@@ -107,11 +112,11 @@ struct ViewSet {
 
 struct StepPlanKey {
   int64_t n;
-  int deg, nviews, mode, tw, batch, persistent;
+  int deg, nviews, mode, tw, batch, persistent, densify;
   std::vector<uint32_t> dims;
   bool operator<(const StepPlanKey& o) const {
-    return std::tie(n, deg, nviews, mode, tw, batch, persistent, dims) <
-           std::tie(o.n, o.deg, o.nviews, o.mode, o.tw, o.batch, o.persistent, o.dims);
+    return std::tie(n, deg, nviews, mode, tw, batch, persistent, densify, dims) <
+           std::tie(o.n, o.deg, o.nviews, o.mode, o.tw, o.batch, o.persistent, o.densify, o.dims);
   }
   bool operator==(const StepPlanKey& o) const { return !(*this < o) && !(o < *this); }
 };
@@ -307,6 +312,7 @@ int ensure_workspace(rgs_ctx* c, const rgs_splats* s, int view_slots, uint32_t t
     rc |= ensure(c, w.sk2, ci * 4);
     rc |= ensure(c, w.sv2, ci * 4);
     rc |= ensure(c, w.offs, ci * 4);
+    rc |= ensure(c, w.visi, ci + 4);
     bool acc_new = w.acc.bytes < ci * kAcc * 4;
     rc |= ensure(c, w.acc, ci * kAcc * 4);
     if (rc) return RGS_ERR_HIP;
@@ -376,7 +382,8 @@ uint32_t raster_grid(rgs_ctx* c, bool bwd, uint32_t tiles) {
 }
 
 // Enqueue the forward of views [v0, v0+nv) (render_forward, forward/mod.rs:156-294).
-int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, int nv, float blur, bool with_loss) {
+int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, int nv, float blur, bool with_loss,
+                    bool vis_inst = false) {
   Workspace& w = c->ws;
   hipStream_t st = c->stream;
   const uint32_t n = (uint32_t)s->n;
@@ -457,6 +464,7 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
   A.cull = c->no_cull ? 0 : 1;
   A.n = n;
   A.visible = s->visible.as<float>();
+  A.vis_inst = vis_inst ? w.visi.as<uint8_t>() : nullptr;
   A.loss = with_loss ? w.fscal.as<float>() + kFLossBase : nullptr;
   A.acc = nullptr;
   A.frag = (c->count_frags && !c->capturing) ? c->frag.as<unsigned long long>() : nullptr;
@@ -474,7 +482,7 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
 }
 
 int enqueue_backward(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, int v0, int nv, float blur, bool accumulate,
-                     bool refine) {
+                     bool refine, bool densify = false) {
   Workspace& w = c->ws;
   hipStream_t st = c->stream;
   const uint32_t n = (uint32_t)s->n;
@@ -513,7 +521,8 @@ int enqueue_backward(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, int v0, int n
   }
   StageTimer tp(c, RGS_STAGE_PROJECT_BWD);
   launch_project_bwd(s->deg, st, s->P.as<float>(), s->npad, n, w.views.as<ViewDev>(), v0, nv, blur, w.acc.as<float>(),
-                     w.keys.as<uint32_t>(), s->G.as<float>(), s->stats.as<float>(), accumulate ? 1 : 0, counters);
+                     w.keys.as<uint32_t>(), s->G.as<float>(), s->stats.as<float>(), accumulate ? 1 : 0, counters,
+                     densify ? s->dinc.as<float>() : nullptr, densify ? w.visi.as<uint8_t>() : nullptr);
   return RGS_OK;
 }
 
@@ -537,24 +546,29 @@ int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cf
                  const std::vector<int>* groups) {
   const int nv = (int)vs.v.size();
   const float blur = cfg->cov_blur > 0.0f ? cfg->cov_blur : 0.0f;
+  const bool dens = (cfg->densify & RGS_DENSIFY_ACCUMULATE) != 0;
   launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
+  if (dens) launch_fill_u32(c->stream, c->ws.visi.as<uint32_t>(), 0u, ((size_t)s->n * nv + 3) / 4);
   int rc = RGS_OK;
   if (cfg->mode == RGS_MODE_BATCHED) {
     const std::vector<int> all = {0, nv};
     const std::vector<int>& g = (groups && groups->size() >= 2) ? *groups : all;
     for (size_t i = 0; i + 1 < g.size(); ++i) {
       const int v0 = g[i], cnt = g[i + 1] - g[i];
-      if ((rc = enqueue_forward(c, s, vs, v0, cnt, blur, true))) return rc;
-      if ((rc = enqueue_backward(c, s, vs, v0, cnt, blur, i > 0, false))) return rc;
+      if ((rc = enqueue_forward(c, s, vs, v0, cnt, blur, true, dens))) return rc;
+      if ((rc = enqueue_backward(c, s, vs, v0, cnt, blur, i > 0, false, dens))) return rc;
     }
   } else {
     for (int v = 0; v < nv; ++v) {
-      if ((rc = enqueue_forward(c, s, vs, v, 1, blur, true))) return rc;
-      if ((rc = enqueue_backward(c, s, vs, v, 1, blur, v > 0, false))) return rc;
+      if ((rc = enqueue_forward(c, s, vs, v, 1, blur, true, dens))) return rc;
+      if ((rc = enqueue_backward(c, s, vs, v, 1, blur, v > 0, false, dens))) return rc;
     }
   }
   enqueue_adam(c, s, &cfg->adam, nv, cfg->check_loss);
+  if (dens)
+    launch_densify_commit(c->stream, s->dens.as<float>(), s->dinc.as<float>(), s->stats.as<float>(), s->npad,
+                          (uint32_t)s->n, nv, cfg->densify, c->ws.counters.as<uint32_t>());
   launch_step_end(c->stream, c->ws.counters.as<uint32_t>());
   HIPCHK(c, hipMemcpyAsync(c->h_step, c->ws.counters.p, sizeof(uint32_t) * kCntNum, hipMemcpyDeviceToHost,
                            c->stream));
@@ -655,6 +669,7 @@ StepPlanKey plan_key(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int mod
   k.tw = (int)c->raster.tile_width;
   k.batch = (int)c->raster.batch_depth;
   k.persistent = (int)c->raster.persistent;
+  k.densify = 0;
   for (auto& v : vs.v) {
     k.dims.push_back(v.W);
     k.dims.push_back(v.H);
@@ -709,7 +724,7 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   Workspace& w = c->ws;
   release(c->frag);
   DevBuf* bufs[] = {&w.views, &w.proj, &w.keys, &w.vals, &w.cnt, &w.scnt, &w.sk1, &w.sv1, &w.sk2, &w.sv2, &w.offs, &w.ik0,
-                    &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch};
+                    &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch, &w.visi};
   for (DevBuf* b : bufs) release(*b);
   for (auto& b : w.own_img) release(b);
   hipHostFree(c->h_counters);
@@ -781,6 +796,8 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   rc |= ensure(c, s->visible, (size_t)s->npad * 4);
   rc |= ensure(c, s->lr, 4 * (size_t)s->nplanes);
   rc |= ensure(c, s->step, 4 * kAdamGroups);
+  rc |= ensure(c, s->dens, (size_t)s->npad * 4 * 10);
+  rc |= ensure(c, s->dinc, (size_t)s->npad * 4 * 5);
   if (rc) {
     rgs_splats_destroy(c, s);
     return RGS_ERR_HIP;
@@ -794,6 +811,8 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
   launch_fill_f32(c->stream, s->lr.as<float>(), 0.0f, s->nplanes);
   launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, kAdamGroups);
+  launch_fill_f32(c->stream, s->dens.as<float>(), 0.0f, (size_t)s->npad * 10);
+  launch_fill_f32(c->stream, s->dinc.as<float>(), 0.0f, (size_t)s->npad * 5);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rgs_splats_destroy(c, s);
     return fail(c, RGS_ERR_HIP, "stream error");
@@ -809,7 +828,7 @@ void rgs_splats_destroy(rgs_ctx* c, rgs_splats* s) {
     if (c->gsplats == s) invalidate_graph(c);
     if (c->fwd_splats == s) c->fwd_valid = false;
   }
-  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr, &s->step};
+  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr, &s->step, &s->dens, &s->dinc};
   for (DevBuf* b : bufs) release(*b);
   delete s;
 }
@@ -948,6 +967,22 @@ int rgs_adam_import_moments(rgs_ctx* c, rgs_splats* s, const float* m_tf, const 
   return rc;
 }
 
+int rgs_densify_export(rgs_ctx* c, const rgs_splats* s, float* out) {
+  if (!c || !s || !out) return RGS_ERR_INVALID;
+  for (int k = 0; k < 10; ++k)
+    HIPCHK(c, hipMemcpyAsync(out + (size_t)k * s->n, s->dens.as<float>() + (size_t)k * s->npad, 4 * (size_t)s->n,
+                             hipMemcpyDefault, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RGS_OK;
+}
+
+int rgs_densify_reset(rgs_ctx* c, rgs_splats* s) {
+  if (!c || !s) return RGS_ERR_INVALID;
+  launch_fill_f32(c->stream, s->dens.as<float>(), 0.0f, (size_t)s->npad * 10);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RGS_OK;
+}
+
 int rgs_splats_export_visible(rgs_ctx* c, const rgs_splats* s, float* visible) {
   if (!c || !s || !visible) return RGS_ERR_INVALID;
   HIPCHK(c, hipMemcpyAsync(visible, s->visible.p, (size_t)s->n * 4, hipMemcpyDefault, c->stream));
@@ -1072,7 +1107,8 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
   std::vector<float> lr(s->nplanes);
   set_lr(s, &cfg->adam, lr.data());
   HIPCHK(c, hipMemcpyAsync(s->lr.p, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice, c->stream));
-  const StepPlanKey key = plan_key(c, s, vs, cfg->mode);
+  StepPlanKey key = plan_key(c, s, vs, cfg->mode);
+  key.densify = cfg->densify;
   // An asynchronous step leaves its counters for the next call.  Waiting for
   // the previous step here keeps one step in flight, so a capacity overflow
   // (which skips that step's update) is seen before the next step is

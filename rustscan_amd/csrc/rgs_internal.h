@@ -89,6 +89,7 @@ struct RasterArgs {
   int cull;         // nonzero: skip splats per 8x8 block that provably miss it
   uint32_t n;       // Gaussians per view; instance = view * n + gid
   float* visible;   // [n] OR over views, or null
+  uint8_t* vis_inst;  // [instances] 1 where a fragment blended the instance, or null
   float* loss;      // per-view sums of |pred - target|, or null
   float* acc;       // backward: [instances][16]
   // optional fragment counters: forward [0] evaluated, [1] blended;

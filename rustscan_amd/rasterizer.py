@@ -186,6 +186,15 @@ class DeviceSplats:
         arrs = [np.ascontiguousarray(a, np.float32) for a in (*m, *v)]
         self.ctx.check(self.ctx.lib.rgs_adam_import_moments(self.ctx.h, self.h, *[a.ctypes.data for a in arrs]))
 
+    def densify_stats(self) -> dict:
+        """The trainer's densification accumulators (trainer.rs:66-75), [N] each."""
+        out = np.zeros((10, self.n), np.float32)
+        self.ctx.check(self.ctx.lib.rgs_densify_export(self.ctx.h, self.h, out.ctypes.data))
+        return {name: out[i] for i, name in enumerate(L.DENSIFY_FIELDS)}
+
+    def reset_densify_stats(self):
+        self.ctx.check(self.ctx.lib.rgs_densify_reset(self.ctx.h, self.h))
+
     def visible(self) -> np.ndarray:
         out = np.zeros(self.n, np.float32)
         self.ctx.check(self.ctx.lib.rgs_splats_export_visible(self.ctx.h, self.h, out.ctypes.data))
@@ -320,7 +329,9 @@ class Trainer:
     """Fused training step (WgpuTrainer::train_step) over one or more views."""
 
     def __init__(self, ctx: Context, optimizer: AdamScaled, cov_blur: float = 0.3, background=(0.0, 0.0, 0.0),
-                 loss_weight: float = 1.0, check_loss: bool = True):
+                 loss_weight: float = 1.0, check_loss: bool = True, densify: int = 0):
+        """densify: RGS_DENSIFY_* flags (accumulate_gradients each step, trainer.rs:651-728)."""
+        self.densify = densify
         self.ctx = ctx
         self.optimizer = optimizer
         self.cov_blur = cov_blur
@@ -338,6 +349,7 @@ class Trainer:
         c.loss_weight = float(self.loss_weight)
         c.check_loss = 1 if self.check_loss else 0
         c.adam = self.optimizer.to_cfg()
+        c.densify = int(self.densify)
         return c
 
     def train_step(self, splats: DeviceSplats, cams, targets, batched: bool = True, graph: bool = False,
