@@ -254,6 +254,26 @@ def main() -> int:
     pv_sps = pv_steps / t_pv
     log(f"per-view: {pv_sps:.2f} steps/s ({1e3 * t_pv / pv_steps:.2f} ms/step)")
 
+    # ---- the same batched step with RustGS's default loss (L1 0.8 + D-SSIM 0.2,
+    # loss.rs:112-142, config.rs:595-608) in place of the configurations' L1
+    from rustscan_amd.rasterizer import LossConfig
+
+    state0 = splats.download()
+    ssim_trainer = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams), densify=RGS_DENSIFY_ACCUMULATE,
+                           loss=LossConfig())
+    for _ in range(args.warmup):
+        ssim_trainer.train_step(splats, cams, targets, batched=True, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ssim_trainer.train_step(splats, cams, targets, batched=True, graph=True)
+    ctx.sync()
+    t_ssim = time.perf_counter() - t0
+    default_loss = {"loss": "L1 0.8 + D-SSIM 0.2", "steps_per_s": round(args.steps / t_ssim, 2),
+                    "ms_per_step": round(1e3 * t_ssim / args.steps, 3)}
+    log(f"batched+graph with L1 0.8 + D-SSIM 0.2: {default_loss}")
+    splats.upload(state0)
+
     # ---- gradient agreement batched vs per-view from identical parameters
     state = splats.download()
     splats.upload(state)
@@ -476,6 +496,7 @@ def main() -> int:
         "grad_rel_err_vs_oracle_per_group": grad_err_groups,
         "gpu_mpix_per_s": mpix,
         "skipped_steps": skipped,
+        "default_loss_step": default_loss,
         "stages": stage_summary,
         "counts": {"visible": int(sum(c[0] for c in counts)), "intersections": int(sum(c[1] for c in counts)),
                    "tiles": int(sum(c[2] for c in counts))} if counts else None,

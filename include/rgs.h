@@ -73,6 +73,20 @@ enum { RGS_GRAPH_OFF = 0, RGS_GRAPH_ON = 1 };
  * engine/trainer.rs:253-440, generalised to several views per step):
  * forward, L1 loss with weight loss_weight per view (loss.rs:144-196,
  * mean over H*W*3), backward summed over views, one Adam update. */
+/* The training loss (combined_loss_with_kernel, engine/loss.rs:112-142, with
+ * TrainingLossConfig, config.rs:566-608).  combined = 0: plain L1 (the
+ * configurations' loss; fused into the compositing kernels).  combined = 1:
+ * l1_weight * residual (L1; saturating when robust_delta > 0; soft outlier
+ * weighting when outlier_threshold > 0 and outlier_weight < 1) + ssim_weight
+ * * (1 - mean SSIM, 11-tap Gaussian window, sigma 1.5) + gradient_weight *
+ * image-gradient difference.  The dynamic residual mask (loss.rs:177-211) is
+ * not supported.  RustGS's defaults: combined, 0.8, 0.2, 0, 0, 0, 1. */
+typedef struct rgs_loss_cfg {
+  int combined;
+  float l1_weight, ssim_weight, gradient_weight;
+  float robust_delta, outlier_threshold, outlier_weight;
+} rgs_loss_cfg;
+
 typedef struct rgs_step_cfg {
   int mode;          /* RGS_MODE_PER_VIEW: views one after another, grads
                         accumulated; RGS_MODE_BATCHED: all views per launch
@@ -89,6 +103,7 @@ typedef struct rgs_step_cfg {
   int densify;       /* RGS_DENSIFY_* flags: accumulate the densification
                         statistics of each view (accumulate_gradients,
                         trainer.rs:651-728; see rgs_densify_export) */
+  rgs_loss_cfg loss; /* loss of each view, times loss_weight */
 } rgs_step_cfg;
 
 enum {
@@ -198,6 +213,13 @@ int rgs_render_backward(rgs_ctx* ctx, rgs_splats* s, const float* const* v_out);
  * when loss_out is non-NULL. */
 int rgs_l1_loss_grad(rgs_ctx* ctx, const float* pred, const float* target, uint32_t width, uint32_t height,
                      float weight, float* v_out, float* loss_out);
+
+/* ---- combined loss (combined_loss_with_kernel, loss.rs:112-142) ----------
+ * pred [H][W][4], target [H][W][3], v_out [H][W][4] (device; v_out may be
+ * NULL); *loss_out (host, may be NULL) = weight * loss.  v_out = weight *
+ * d loss / d pred on RGB, 0 on alpha.  Synchronous. */
+int rgs_image_loss_grad(rgs_ctx* ctx, const float* pred, const float* target, uint32_t width, uint32_t height,
+                        float weight, const rgs_loss_cfg* cfg, float* v_out, float* loss_out);
 
 /* ---- Adam: AdamScaled::step_device_splats (optimizer.rs:80-120) -----------
  * One update of all parameter groups from the splat store's gradients. */

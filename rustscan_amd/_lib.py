@@ -69,6 +69,18 @@ class RgsAdamCfg(ctypes.Structure):
     ]
 
 
+class RgsLossCfg(ctypes.Structure):
+    _fields_ = [
+        ("combined", c_int),
+        ("l1_weight", c_float),
+        ("ssim_weight", c_float),
+        ("gradient_weight", c_float),
+        ("robust_delta", c_float),
+        ("outlier_threshold", c_float),
+        ("outlier_weight", c_float),
+    ]
+
+
 class RgsStepCfg(ctypes.Structure):
     _fields_ = [
         ("mode", c_int),
@@ -79,6 +91,7 @@ class RgsStepCfg(ctypes.Structure):
         ("check_loss", c_int),
         ("adam", RgsAdamCfg),
         ("densify", c_int),
+        ("loss", RgsLossCfg),
     ]
 
 
@@ -127,6 +140,8 @@ _SIGNATURES = {
     "rgs_l1_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, c_void_p,
                                  POINTER(c_float)]),
     "rgs_adam_step": (c_int, [c_void_p, c_void_p, POINTER(RgsAdamCfg)]),
+    "rgs_image_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, POINTER(RgsLossCfg),
+                                    c_void_p, POINTER(c_float)]),
     "rgs_train_step": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_void_p), c_int,
                                POINTER(RgsStepCfg), POINTER(c_float)]),
     "rgs_autotune": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_void_p), c_int, POINTER(RgsStepCfg),

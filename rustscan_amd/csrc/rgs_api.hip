@@ -60,6 +60,10 @@ void launch_aos_to_soa(hipStream_t s, const float* aos, int width, uint32_t n, f
 void launch_soa_to_aos(hipStream_t s, const float* soa, uint32_t npad, int plane0, int width, uint32_t n, float* aos);
 void launch_fill_f32(hipStream_t s, float* p, float v, size_t n);
 hipError_t noalloc_setup(int device);
+void make_loss_coef(const float* w6, int residual, int degenerate_nan, LossCoef* L);
+uint32_t loss_blocks(uint32_t W, uint32_t H);
+void launch_loss(hipStream_t s, const ViewDev* views, int v0, int nv, uint32_t max_blocks, const LossCoef& L,
+                 float* loss_sums);
 int noalloc_sort_pairs(int device, hipStream_t stream, const uint32_t* keys, const uint32_t* values,
                        uint32_t* out_keys, uint32_t* out_values, uint32_t n, const char** err);
 }  // namespace rgs
@@ -102,6 +106,8 @@ struct Workspace {
   DevBuf views, proj, keys, vals, cnt, scnt, sk1, sv1, sk2, sv2, offs, ik0, iv0, ik1, iv1, ranges, acc, acc_dbg, counters,
       fscal, scratch, visi;
   std::vector<DevBuf> own_img;  // images kept internal when the caller passes none
+  // combined loss: per view, the SSIM partial-derivative maps and the image gradient
+  std::vector<DevBuf> loss_grad, loss_vout;
 };
 
 struct ViewSet {
@@ -114,9 +120,10 @@ struct StepPlanKey {
   int64_t n;
   int deg, nviews, mode, tw, batch, persistent, densify;
   std::vector<uint32_t> dims;
+  std::vector<uint32_t> loss;  // bits of the loss configuration (captured as kernel arguments)
   bool operator<(const StepPlanKey& o) const {
-    return std::tie(n, deg, nviews, mode, tw, batch, persistent, densify, dims) <
-           std::tie(o.n, o.deg, o.nviews, o.mode, o.tw, o.batch, o.persistent, o.densify, o.dims);
+    return std::tie(n, deg, nviews, mode, tw, batch, persistent, densify, dims, loss) <
+           std::tie(o.n, o.deg, o.nviews, o.mode, o.tw, o.batch, o.persistent, o.densify, o.dims, o.loss);
   }
   bool operator==(const StepPlanKey& o) const { return !(*this < o) && !(o < *this); }
 };
@@ -160,6 +167,8 @@ struct rgs_ctx {
   hipEvent_t step_done = nullptr;
   bool step_pending = false;
   std::map<std::vector<uint32_t>, rgs_raster_cfg> tuned;
+  // the single view of rgs_image_loss_grad (kept apart from the forward state)
+  DevBuf loss_view;
   // per-stage timing
   bool timing = false;
   bool capturing = false;
@@ -285,6 +294,8 @@ int build_views(rgs_ctx* c, const rgs_camera* cams, int nv, const float* bg, flo
     V.img = nullptr;
     V.target = nullptr;
     V.vout = nullptr;
+    V.lgrad = nullptr;
+    V.loss_weight = loss_weight;
     tile_base += V.tiles_x * V.tiles_y;
     pix += (uint64_t)V.W * V.H;
   }
@@ -363,6 +374,45 @@ int attach_images(rgs_ctx* c, ViewSet& vs, float* const* out_rgba) {
       if (ensure(c, w.own_img[i], (size_t)vs.v[i].W * vs.v[i].H * 16)) return RGS_ERR_HIP;
       vs.v[i].img = w.own_img[i].as<float>();
     }
+  }
+  return RGS_OK;
+}
+
+// TrainingLossConfig -> kernel coefficients; the residual variant follows
+// reconstruction_residual_loss (loss.rs:153-176).
+LossCoef loss_coef(const rgs_loss_cfg& cfg, bool degenerate_nan = true) {
+  int mode = 0;
+  if (std::isfinite(cfg.robust_delta) && cfg.robust_delta > 0.0f)
+    mode = 1;
+  else if (std::isfinite(cfg.outlier_threshold) && cfg.outlier_threshold > 0.0f &&
+           std::isfinite(cfg.outlier_weight) && cfg.outlier_weight < 1.0f)
+    mode = 2;
+  const float w6[6] = {cfg.l1_weight, cfg.ssim_weight, cfg.gradient_weight, cfg.robust_delta,
+                       cfg.outlier_threshold, cfg.outlier_weight};
+  LossCoef L;
+  make_loss_coef(w6, mode, degenerate_nan ? 1 : 0, &L);
+  return L;
+}
+
+int check_loss_cfg(rgs_ctx* c, const rgs_loss_cfg& cfg) {
+  const float w[6] = {cfg.l1_weight, cfg.ssim_weight, cfg.gradient_weight, cfg.robust_delta,
+                      cfg.outlier_threshold, cfg.outlier_weight};
+  for (float x : w)
+    if (!std::isfinite(x) || x < 0.0f) return fail(c, RGS_ERR_INVALID, "loss weights must be finite and >= 0");
+  return RGS_OK;
+}
+
+// Per-view buffers of the combined loss: SSIM partial-derivative maps and the
+// image gradient the compositing backward reads.
+int attach_loss_buffers(rgs_ctx* c, ViewSet& vs) {
+  Workspace& w = c->ws;
+  if (w.loss_grad.size() < vs.v.size()) w.loss_grad.resize(vs.v.size());
+  if (w.loss_vout.size() < vs.v.size()) w.loss_vout.resize(vs.v.size());
+  for (size_t i = 0; i < vs.v.size(); ++i) {
+    const size_t np = (size_t)vs.v[i].W * vs.v[i].H;
+    if (ensure(c, w.loss_grad[i], np * 9 * 4) || ensure(c, w.loss_vout[i], np * 16)) return RGS_ERR_HIP;
+    vs.v[i].lgrad = w.loss_grad[i].as<float>();
+    vs.v[i].vout = w.loss_vout[i].as<float>();
   }
   return RGS_OK;
 }
@@ -547,6 +597,15 @@ int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cf
   const int nv = (int)vs.v.size();
   const float blur = cfg->cov_blur > 0.0f ? cfg->cov_blur : 0.0f;
   const bool dens = (cfg->densify & RGS_DENSIFY_ACCUMULATE) != 0;
+  const bool combined = cfg->loss.combined != 0;
+  const LossCoef lcoef = loss_coef(cfg->loss);
+  // the combined loss runs between a group's forward and its backward
+  auto enqueue_loss = [&](int v0, int cnt) {
+    if (!combined) return;
+    uint32_t blocks = 0;
+    for (int v = v0; v < v0 + cnt; ++v) blocks = std::max(blocks, loss_blocks(vs.v[v].W, vs.v[v].H));
+    launch_loss(c->stream, c->ws.views.as<ViewDev>(), v0, cnt, blocks, lcoef, c->ws.fscal.as<float>() + kFLossBase);
+  };
   launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
   if (dens) launch_fill_u32(c->stream, c->ws.visi.as<uint32_t>(), 0u, ((size_t)s->n * nv + 3) / 4);
@@ -556,12 +615,14 @@ int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cf
     const std::vector<int>& g = (groups && groups->size() >= 2) ? *groups : all;
     for (size_t i = 0; i + 1 < g.size(); ++i) {
       const int v0 = g[i], cnt = g[i + 1] - g[i];
-      if ((rc = enqueue_forward(c, s, vs, v0, cnt, blur, true, dens))) return rc;
+      if ((rc = enqueue_forward(c, s, vs, v0, cnt, blur, !combined, dens))) return rc;
+      enqueue_loss(v0, cnt);
       if ((rc = enqueue_backward(c, s, vs, v0, cnt, blur, i > 0, false, dens))) return rc;
     }
   } else {
     for (int v = 0; v < nv; ++v) {
-      if ((rc = enqueue_forward(c, s, vs, v, 1, blur, true, dens))) return rc;
+      if ((rc = enqueue_forward(c, s, vs, v, 1, blur, !combined, dens))) return rc;
+      enqueue_loss(v, 1);
       if ((rc = enqueue_backward(c, s, vs, v, 1, blur, v > 0, false, dens))) return rc;
     }
   }
@@ -727,6 +788,9 @@ void rgs_ctx_destroy(rgs_ctx* c) {
                     &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch, &w.visi};
   for (DevBuf* b : bufs) release(*b);
   for (auto& b : w.own_img) release(b);
+  for (auto& b : w.loss_grad) release(b);
+  for (auto& b : w.loss_vout) release(b);
+  release(c->loss_view);
   hipHostFree(c->h_counters);
   hipHostFree(c->h_fscal);
   hipHostFree(c->h_step);
@@ -1074,6 +1138,49 @@ int rgs_l1_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint32_
   return RGS_OK;
 }
 
+int rgs_image_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint32_t W, uint32_t H, float weight,
+                        const rgs_loss_cfg* cfg, float* v_out, float* loss_out) {
+  if (!c || !pred || !target || !cfg || W == 0 || H == 0 || W > 16384 || H > 16384) return RGS_ERR_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = check_loss_cfg(c, *cfg);
+  if (rc) return rc;
+  rgs_loss_cfg lc = *cfg;
+  const bool plain = !lc.combined;
+  if (plain) lc = rgs_loss_cfg{1, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};  // plain L1 through the same kernels
+  const size_t np = (size_t)W * H;
+  DevBuf grad, vtmp, acc;
+  if (ensure(c, grad, np * 9 * 4) || ensure(c, acc, 4) || (!v_out && ensure(c, vtmp, np * 16)) ||
+      ensure(c, c->loss_view, sizeof(ViewDev))) {
+    release(grad);
+    release(vtmp);
+    release(acc);
+    return RGS_ERR_HIP;
+  }
+  ViewDev V{};
+  V.W = W;
+  V.H = H;
+  V.img = const_cast<float*>(pred);
+  V.target = target;
+  V.vout = v_out ? v_out : vtmp.as<float>();
+  V.lgrad = grad.as<float>();
+  V.loss_weight = weight;
+  hipError_t e = hipMemcpyAsync(c->loss_view.p, &V, sizeof(V), hipMemcpyHostToDevice, c->stream);
+  launch_fill_f32(c->stream, acc.as<float>(), 0.0f, 1);
+  if (e == hipSuccess) {
+    launch_loss(c->stream, c->loss_view.as<ViewDev>(), 0, 1, loss_blocks(W, H), loss_coef(lc, !plain),
+                acc.as<float>());
+    float sum = 0.0f;
+    e = hipMemcpyAsync(&sum, acc.p, 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && loss_out) *loss_out = weight * sum / (float)(3ull * np);
+  }
+  release(grad);
+  release(vtmp);
+  release(acc);
+  if (e != hipSuccess) return fail(c, RGS_ERR_HIP, "loss: %s", hipGetErrorString(e));
+  return RGS_OK;
+}
+
 int rgs_adam_step(rgs_ctx* c, rgs_splats* s, const rgs_adam_cfg* cfg) {
   if (!c || !s || !cfg) return RGS_ERR_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1101,6 +1208,10 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
     vs.v[i].target = targets[i];
   }
   if ((rc = attach_images(c, vs, nullptr))) return rc;
+  if (cfg->loss.combined) {
+    if ((rc = check_loss_cfg(c, cfg->loss))) return rc;
+    if ((rc = attach_loss_buffers(c, vs))) return rc;
+  }
   if ((rc = ensure_workspace(c, s, nv, vs.tiles_total, 0))) return rc;
   const float blur = cfg->cov_blur > 0.0f ? cfg->cov_blur : 0.0f;
   // learning rates (device table; updated every call so schedules can change them)
@@ -1109,6 +1220,16 @@ int rgs_train_step(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, const floa
   HIPCHK(c, hipMemcpyAsync(s->lr.p, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice, c->stream));
   StepPlanKey key = plan_key(c, s, vs, cfg->mode);
   key.densify = cfg->densify;
+  {
+    const float lw[6] = {cfg->loss.l1_weight, cfg->loss.ssim_weight, cfg->loss.gradient_weight,
+                         cfg->loss.robust_delta, cfg->loss.outlier_threshold, cfg->loss.outlier_weight};
+    key.loss.assign(1, (uint32_t)cfg->loss.combined);
+    for (float x : lw) {
+      uint32_t b;
+      std::memcpy(&b, &x, 4);
+      key.loss.push_back(cfg->loss.combined ? b : 0u);
+    }
+  }
   // An asynchronous step leaves its counters for the next call.  Waiting for
   // the previous step here keeps one step in flight, so a capacity overflow
   // (which skips that step's update) is seen before the next step is

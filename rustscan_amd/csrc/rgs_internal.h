@@ -30,6 +30,10 @@ struct ViewDev {
   float* img;
   const float* target;
   const float* vout;
+  // combined loss: [9][H][W] partial derivatives of the SSIM map (channel c,
+  // map m at plane 3c + m), written by the loss forward, read by its backward
+  float* lgrad;
+  float loss_weight;
 };
 
 // Projected splat instance, 64 bytes, the unit every later stage gathers by
@@ -75,6 +79,20 @@ constexpr int kAdamGroups = 3;
 __host__ __device__ inline int adam_group_of_plane(int plane) { return plane < 10 ? 0 : (plane == 10 ? 2 : 1); }
 
 constexpr int kAcc = 16;
+
+// Coefficients of the combined loss kernels (k_loss.hip), passed by value.
+struct LossCoef {
+  float l1, ssim, grad;  // term weights
+  int residual;          // 0: L1, 1: saturating robust, 2: soft outlier
+  int degenerate_nan;    // the loss of an image one pixel wide or high is NaN (the reference's
+                         // image-gradient term averages an empty difference there)
+  float delta;           // robust delta, or outlier threshold
+  float floor_w;         // outlier gradient floor
+  float c1, c2;          // SSIM constants (k1 * range)^2, (k2 * range)^2
+  float k[11];           // gaussian_kernel_1d
+  float kcum[11];        // k[0] + ... + k[m]
+  float kdown[11];       // k[m] + ... + k[10]
+};
 
 // Arguments of the compositing kernels.  Work items are global tile ids
 // [tile_first, tile_first + tile_count).

@@ -244,6 +244,47 @@ def l1_loss_grad(ctx: Context, pred, target, weight: float = 1.0):
 
 
 @dataclass
+class LossConfig:
+    """TrainingLossConfig (training/config.rs:566-608) fields used by
+    combined_loss_with_kernel (engine/loss.rs:112-142); the defaults are
+    RustGS's.  LossConfig.l1() is the plain L1 loss of the benchmark
+    configurations."""
+
+    l1_weight: float = 0.8
+    ssim_weight: float = 0.2
+    gradient_weight: float = 0.0
+    robust_delta: float = 0.0
+    outlier_threshold: float = 0.0
+    outlier_weight: float = 1.0
+    combined: bool = True
+
+    @staticmethod
+    def l1() -> "LossConfig":
+        return LossConfig(1.0, 0.0, 0.0, 0.0, 0.0, 1.0, combined=False)
+
+    def to_cfg(self) -> L.RgsLossCfg:
+        return L.RgsLossCfg(1 if self.combined else 0, self.l1_weight, self.ssim_weight, self.gradient_weight,
+                            self.robust_delta, self.outlier_threshold, self.outlier_weight)
+
+
+def image_loss_grad(ctx: Context, pred, target, weight: float = 1.0, cfg: LossConfig | None = None):
+    """combined_loss_with_kernel of pred [H,W,4] against target [H,W,3]:
+    (weight * loss, weight * d loss / d pred as an [H,W,4] device tensor)."""
+    torch = _torch()
+    cfg = cfg or LossConfig()
+    h, w = int(pred.shape[0]), int(pred.shape[1])
+    v_out = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    loss = c_float()
+    c = cfg.to_cfg()
+    pred = pred.contiguous()
+    target = target.contiguous()
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.rgs_image_loss_grad(ctx.h, pred.data_ptr(), target.data_ptr(), w, h, float(weight),
+                                          ctypes.byref(c), v_out.data_ptr(), ctypes.byref(loss)))
+    return loss.value, v_out
+
+
+@dataclass
 class AdamScaledConfig:
     """engine/optimizer.rs:7-24 (lr 1.0 in the trainer, per-column scaling)."""
 
@@ -329,9 +370,11 @@ class Trainer:
     """Fused training step (WgpuTrainer::train_step) over one or more views."""
 
     def __init__(self, ctx: Context, optimizer: AdamScaled, cov_blur: float = 0.3, background=(0.0, 0.0, 0.0),
-                 loss_weight: float = 1.0, check_loss: bool = True, densify: int = 0):
-        """densify: RGS_DENSIFY_* flags (accumulate_gradients each step, trainer.rs:651-728)."""
+                 loss_weight: float = 1.0, check_loss: bool = True, densify: int = 0, loss: LossConfig | None = None):
+        """densify: RGS_DENSIFY_* flags (accumulate_gradients each step, trainer.rs:651-728);
+        loss: the loss of each view (default: plain L1)."""
         self.densify = densify
+        self.loss = loss or LossConfig.l1()
         self.ctx = ctx
         self.optimizer = optimizer
         self.cov_blur = cov_blur
@@ -350,6 +393,7 @@ class Trainer:
         c.check_loss = 1 if self.check_loss else 0
         c.adam = self.optimizer.to_cfg()
         c.densify = int(self.densify)
+        c.loss = self.loss.to_cfg()
         return c
 
     def train_step(self, splats: DeviceSplats, cams, targets, batched: bool = True, graph: bool = False,
