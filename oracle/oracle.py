@@ -394,3 +394,140 @@ class DensifyAccumulators:
         if collect_actual_visibility:
             a["actual_visible_observations"] += vis
         a["visible_observations"] += vis if use_actual_visibility else f(1.0)
+
+
+# ---- topology step (oracle/topology.cpp) ------------------------------------
+class OrcTopologyCfg(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("max_initial_gaussians", ctypes.c_uint32),
+        ("frame_shuffle_seed", ctypes.c_uint64),
+        ("refine_every", ctypes.c_uint32),
+        ("opacity_reset_interval", ctypes.c_uint32),
+        ("opacity_reset_mode", ctypes.c_uint32),
+        ("topology_freeze_after_epoch", ctypes.c_int32),
+        ("growth_freeze_after_epoch", ctypes.c_int32),
+        ("prune_until_epoch", ctypes.c_int32),
+        ("split_score_mode", ctypes.c_uint32),
+        ("growth_stop_iter", ctypes.c_uint32),
+        ("growth_grad_threshold", c_float),
+        ("split_grad_threshold", c_float),
+        ("depth_scale_gamma", c_float),
+        ("growth_select_fraction", c_float),
+        ("opacity_decay", c_float),
+        ("scale_decay", c_float),
+        ("prune_mode", ctypes.c_uint32),
+        ("prune_min_age", ctypes.c_uint32),
+        ("prune_invisible_epochs", ctypes.c_uint32),
+        ("prune_visibility_dry_run", ctypes.c_uint32),
+        ("prune_opacity_threshold", c_float),
+        ("prune_visibility_threshold", c_float),
+        ("prune_high_opacity_threshold", c_float),
+    ]
+
+
+# TrainingConfig defaults the topology step reads (config.rs:262-383, 666-686, 712-721, 790-809)
+TOPOLOGY_DEFAULTS = dict(
+    iterations=30000, max_initial_gaussians=100000, frame_shuffle_seed=0, refine_every=160,
+    opacity_reset_interval=10, opacity_reset_mode=0, topology_freeze_after_epoch=-1, growth_freeze_after_epoch=-1,
+    prune_until_epoch=-1, split_score_mode=0, growth_stop_iter=15000, growth_grad_threshold=0.00014,
+    split_grad_threshold=0.00014, depth_scale_gamma=0.37, growth_select_fraction=0.25, opacity_decay=0.0,
+    scale_decay=0.0, prune_mode=1, prune_min_age=5, prune_invisible_epochs=10, prune_visibility_dry_run=0,
+    prune_opacity_threshold=1.0 / 255.0, prune_visibility_threshold=0.05, prune_high_opacity_threshold=0.8)
+
+
+def topology_cfg(**overrides) -> OrcTopologyCfg:
+    d = dict(TOPOLOGY_DEFAULTS)
+    d.update(overrides)
+    return OrcTopologyCfg(**d)
+
+
+class OrcTopologySchedule(ctypes.Structure):
+    _fields_ = [("completed_epoch", ctypes.c_int32), ("densify", ctypes.c_int32), ("prune", ctypes.c_int32),
+                ("reset_opacity", ctypes.c_int32), ("allow_extra_growth", ctypes.c_int32)]
+
+
+class OrcTopologyOut(ctypes.Structure):
+    _fields_ = [
+        ("schedule", OrcTopologySchedule),
+        ("before", c_int64), ("after", c_int64), ("added", c_int64), ("pruned", c_int64),
+        ("clone_candidates", c_int64), ("split_candidates", c_int64), ("growth_candidates", c_int64),
+        ("prune_candidates", c_int64), ("replacement_count", c_int64), ("extra_growth_count", c_int64),
+        ("selected_count", c_int64), ("mutated", ctypes.c_int32), ("adam_reset", ctypes.c_int32),
+        ("center", c_float * 3), ("scene_extent", c_float), ("brush_extent", c_float),
+        ("selected", c_int64 * 4096),
+        ("transforms", POINTER(c_float)), ("sh", POINTER(c_float)), ("raw_opac", POINTER(c_float)),
+        ("birth", POINTER(ctypes.c_uint32)), ("invisible_windows", POINTER(ctypes.c_uint32)),
+        ("row_src", POINTER(c_int64)), ("row_kind", POINTER(ctypes.c_int)), ("flags", POINTER(ctypes.c_uint8)),
+    ]
+
+
+def _topology_lib():
+    L = lib()
+    if not getattr(L, "_topology_bound", False):
+        L.orc_topology_schedule.argtypes = [POINTER(OrcTopologyCfg), ctypes.c_uint32, ctypes.c_uint32,
+                                            POINTER(OrcTopologySchedule)]
+        L.orc_topology_seed.restype = ctypes.c_uint64
+        L.orc_topology_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_topology_refine_scalar.restype = c_float
+        L.orc_topology_refine_scalar.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_topology_key.restype = c_float
+        L.orc_topology_key.argtypes = [ctypes.c_uint64, c_int, ctypes.c_uint64, c_float]
+        L.orc_topology_step.restype = c_int
+        L.orc_topology_step.argtypes = [POINTER(OrcTopologyCfg), c_int64, c_int] + [c_void_p] * 6 + \
+            [ctypes.c_uint32, ctypes.c_uint32, POINTER(OrcTopologyOut)]
+        L.orc_topology_free.argtypes = [POINTER(OrcTopologyOut)]
+        L.orc_brush_refine_offset.argtypes = [c_void_p, c_void_p, c_float, c_void_p]
+        L.orc_cos_2pi.restype = c_float
+        L.orc_cos_2pi.argtypes = [c_float]
+        L._topology_bound = True
+    return L
+
+
+def topology_schedule(cfg: OrcTopologyCfg, iteration: int, frame_count: int) -> dict:
+    out = OrcTopologySchedule()
+    _topology_lib().orc_topology_schedule(ctypes.byref(cfg), iteration, frame_count, ctypes.byref(out))
+    return {f: getattr(out, f) for f, _ in OrcTopologySchedule._fields_}
+
+
+def brush_refine_offset(rot_wxyz, scale, sample_scalar) -> np.ndarray:
+    r, s = _f32(rot_wxyz), _f32(scale)
+    out = np.zeros(3, np.float32)
+    _topology_lib().orc_brush_refine_offset(r.ctypes.data, s.ctypes.data, float(sample_scalar), out.ctypes.data)
+    return out
+
+
+def topology_step(cfg: OrcTopologyCfg, splats, dens, birth, invis, iteration: int, frame_count: int) -> dict:
+    """One topology step of the trainer (apply_topology_mutations,
+    engine/trainer.rs:792-859) on host arrays: dens [10,N] accumulators,
+    birth and invisible-window state [N].  Returns the counts, flags,
+    selection, row sources/kinds, the rebuilt splats and state."""
+    L = _topology_lib()
+    n = splats.n
+    tf, sh, op = _f32(splats.transforms), _f32(splats.sh), _f32(splats.raw_opac)
+    d = _f32(dens).reshape(10, n) if n else np.zeros((10, 0), np.float32)
+    b = np.ascontiguousarray(birth, np.uint32)
+    w = np.ascontiguousarray(invis, np.uint32)
+    out = OrcTopologyOut()
+    rc = L.orc_topology_step(ctypes.byref(cfg), n, splats.sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data,
+                             d.ctypes.data, b.ctypes.data, w.ctypes.data, iteration, frame_count, ctypes.byref(out))
+    if rc:
+        raise ValueError("oracle topology step rejected its arguments")
+    m = out.after
+    k = sh.shape[1] if sh.ndim == 3 else (splats.sh_degree + 1) ** 2
+    res = {f: getattr(out, f) for f, _ in OrcTopologyOut._fields_
+           if f not in ("schedule", "selected", "center", "transforms", "sh", "raw_opac", "birth",
+                        "invisible_windows", "row_src", "row_kind", "flags")}
+    res["schedule"] = {f: getattr(out.schedule, f) for f, _ in OrcTopologySchedule._fields_}
+    res["center"] = np.array(out.center[:], np.float32)
+    res["selected"] = np.array(out.selected[:min(out.selected_count, 4096)], np.int64)
+    res["transforms"] = np.ctypeslib.as_array(out.transforms, (max(m, 1) * 10,))[:m * 10].reshape(m, 10).copy()
+    res["sh"] = np.ctypeslib.as_array(out.sh, (max(m, 1) * k * 3,))[:m * k * 3].reshape(m, k, 3).copy()
+    res["raw_opac"] = np.ctypeslib.as_array(out.raw_opac, (max(m, 1),))[:m].copy()
+    res["birth"] = np.ctypeslib.as_array(out.birth, (max(m, 1),))[:m].copy()
+    res["invisible_windows"] = np.ctypeslib.as_array(out.invisible_windows, (max(m, 1),))[:m].copy()
+    res["row_src"] = np.ctypeslib.as_array(out.row_src, (max(m, 1),))[:m].copy()
+    res["row_kind"] = np.ctypeslib.as_array(out.row_kind, (max(m, 1),))[:m].copy()
+    res["flags"] = np.ctypeslib.as_array(out.flags, (max(n, 1),))[:n].copy()
+    L.orc_topology_free(ctypes.byref(out))
+    return res

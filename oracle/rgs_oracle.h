@@ -143,6 +143,78 @@ void orc_inclusive_scan(int64_t n, const uint32_t* in, uint32_t* out);
 
 int orc_num_threads(void);
 
+
+/* ---- topology step (densify / prune / opacity reset); see topology.cpp ---- */
+/* Same layout as rgs_topology_cfg in include/rgs.h. */
+typedef struct orc_topology_cfg {
+  uint32_t iterations;
+  uint32_t max_initial_gaussians;
+  uint64_t frame_shuffle_seed;
+  uint32_t refine_every;
+  uint32_t opacity_reset_interval;
+  uint32_t opacity_reset_mode;          /* 0 decay, 1 reset */
+  int32_t topology_freeze_after_epoch;  /* -1: none */
+  int32_t growth_freeze_after_epoch;    /* -1: none */
+  int32_t prune_until_epoch;            /* -1: none */
+  uint32_t split_score_mode;            /* 0 baseline, 1 abs, 2 abs-pixel, 3 abs-pixel-depth */
+  uint32_t growth_stop_iter;
+  float growth_grad_threshold;
+  float split_grad_threshold;
+  float depth_scale_gamma;
+  float growth_select_fraction;
+  float opacity_decay;
+  float scale_decay;
+  uint32_t prune_mode;                  /* 0 threshold, 1 weight, 2 visibility-weight */
+  uint32_t prune_min_age;
+  uint32_t prune_invisible_epochs;
+  uint32_t prune_visibility_dry_run;
+  float prune_opacity_threshold;
+  float prune_visibility_threshold;
+  float prune_high_opacity_threshold;
+} orc_topology_cfg;
+
+typedef struct orc_topology_schedule_t {
+  int32_t completed_epoch; /* -1: none */
+  int32_t densify, prune, reset_opacity, allow_extra_growth;
+} orc_topology_schedule_t;
+
+#define ORC_TOPOLOGY_MAX_REPORT 4096
+
+typedef struct orc_topology_out {
+  orc_topology_schedule_t schedule;
+  int64_t before, after, added, pruned;
+  int64_t clone_candidates, split_candidates, growth_candidates, prune_candidates;
+  int64_t replacement_count, extra_growth_count, selected_count;
+  int32_t mutated, adam_reset;
+  float center[3];
+  float scene_extent, brush_extent;
+  int64_t selected[ORC_TOPOLOGY_MAX_REPORT]; /* first selected sources, in order */
+  /* arrays of `after` rows (flags: `before` entries; bit 0 clone, 1 split,
+     2 growth, 3 prune), owned by the oracle until orc_topology_free */
+  float* transforms;
+  float* sh;
+  float* raw_opac;
+  uint32_t* birth;
+  uint32_t* invisible_windows;
+  int64_t* row_src;
+  int* row_kind; /* 0 existing, 1 refined existing, 2 refined new */
+  uint8_t* flags;
+} orc_topology_out;
+
+void orc_topology_schedule(const orc_topology_cfg* cfg, uint32_t iteration, uint32_t frame_count,
+                           orc_topology_schedule_t* out);
+uint64_t orc_topology_seed(uint64_t base_seed, uint32_t iteration);
+float orc_topology_refine_scalar(uint64_t seed, uint64_t j);
+float orc_topology_key(uint64_t seed, int pass, uint64_t i, float w);
+/* dens: [10][n] accumulators (order of rgs_densify_export); birth and
+ * invisible windows per Gaussian.  Returns 0 on success. */
+int orc_topology_step(const orc_topology_cfg* cfg, int64_t n, int deg, const float* transforms, const float* sh,
+                      const float* raw_opac, const float* dens, const uint32_t* birth, const uint32_t* invis,
+                      uint32_t iteration, uint32_t frame_count, orc_topology_out* out);
+void orc_topology_free(orc_topology_out* out);
+float orc_cos_2pi(float u); /* the refine offset's cos(2 pi u) */
+void orc_brush_refine_offset(const float* rot_wxyz, const float* scale, float sample_scalar, float* out);
+
 #ifdef __cplusplus
 }
 #endif

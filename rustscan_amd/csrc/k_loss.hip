@@ -26,6 +26,7 @@ namespace rgs {
 constexpr int kLB = 16;            // output block edge
 constexpr int kLR = 5;             // blur radius (11 taps)
 constexpr int kLS = kLB + 2 * kLR; // staged edge, 26
+constexpr int kLossParts = 64;     // partial loss sums per view
 
 __device__ __forceinline__ void residual_terms(const LossCoef& L, float d, float* r, float* dr) {
   const float a = fabsf(d);
@@ -52,7 +53,7 @@ __device__ __forceinline__ bool view_block(const ViewDev& V, uint32_t* ox, uint3
 }
 
 __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ views, int v0, LossCoef L,
-                                                  float* __restrict__ loss_sums) {
+                                                  float* __restrict__ loss_parts) {
   const int v = v0 + (int)blockIdx.y;
   const ViewDev& V = views[v];
   uint32_t ox, oy;
@@ -61,7 +62,9 @@ __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ vi
   const size_t np = (size_t)W * H;
   const float* __restrict__ pred = V.img;
   const float* __restrict__ tgt = V.target;
-  __shared__ float sx[kLS][kLS + 1], sy[kLS][kLS + 1];
+  // the staged block of all three channels (one 16-byte load per pixel of
+  // the prediction), then one channel at a time through the blur passes
+  __shared__ float sxa[3][kLS][kLS + 1], sya[3][kLS][kLS + 1];
   __shared__ float hb[5][kLS][kLB + 1];
   const int tid = threadIdx.x;
   const int px = tid % kLB, py = tid / kLB;
@@ -69,15 +72,22 @@ __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ vi
   const bool inside = gx < W && gy < H;
   const float N = 3.0f * (float)np;
   float lsum = 0.0f;
+  for (int i = tid; i < kLS * kLS; i += 256) {
+    const int r = i / kLS, col = i % kLS;
+    const int yy = min(max((int)oy + r - kLR, 0), H - 1), xx = min(max((int)ox + col - kLR, 0), W - 1);
+    const size_t p = (size_t)yy * W + xx;
+    const float4 q = reinterpret_cast<const float4*>(pred)[p];
+    sxa[0][r][col] = q.x;
+    sxa[1][r][col] = q.y;
+    sxa[2][r][col] = q.z;
+    sya[0][r][col] = tgt[p * 3];
+    sya[1][r][col] = tgt[p * 3 + 1];
+    sya[2][r][col] = tgt[p * 3 + 2];
+  }
+  __syncthreads();
   for (int c = 0; c < 3; ++c) {
-    for (int i = tid; i < kLS * kLS; i += 256) {
-      const int r = i / kLS, col = i % kLS;
-      const int yy = min(max((int)oy + r - kLR, 0), H - 1), xx = min(max((int)ox + col - kLR, 0), W - 1);
-      const size_t p = (size_t)yy * W + xx;
-      sx[r][col] = pred[p * 4 + c];
-      sy[r][col] = tgt[p * 3 + c];
-    }
-    __syncthreads();
+    float (*sx)[kLS + 1] = sxa[c];
+    float (*sy)[kLS + 1] = sya[c];
     for (int i = tid; i < kLS * kLB; i += 256) {
       const int r = i / kLB, col = i % kLB;
       float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
@@ -138,22 +148,53 @@ __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ vi
     __syncthreads();
   }
   if (L.degenerate_nan && (W == 1 || H == 1) && blockIdx.x == 0 && tid == 0) lsum += __builtin_nanf("");
+  // one partial per block, spread over kLossParts slots per view (one
+  // address per view would serialise the atomics of every block on it)
+  __shared__ float wsum[4];
   const float s = wave_sum(lsum);
-  if ((tid & 63) == 0 && s != 0.0f) atomicAdd(&loss_sums[v], s);
+  if ((tid & 63) == 0) wsum[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) {
+    const float b = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    if (b != 0.0f) atomicAdd(&loss_parts[(size_t)v * kLossParts + (blockIdx.x % kLossParts)], b);
+  }
+}
+
+// Adds each view's partial sums to its loss and clears them for the next use.
+__global__ __launch_bounds__(kLossParts) void k_loss_finish(float* __restrict__ loss_parts, int v0,
+                                                            float* __restrict__ loss_sums) {
+  const int v = v0 + (int)blockIdx.x;
+  float x = loss_parts[(size_t)v * kLossParts + threadIdx.x];
+  loss_parts[(size_t)v * kLossParts + threadIdx.x] = 0.0f;
+  x = wave_sum(x);
+  if (threadIdx.x == 0) loss_sums[v] += x;
 }
 
 // Adjoint of one clamped 11-tap pass at position q of a line of n samples:
 // taps from p in [q-5, q+5] that land on q, plus (at the two ends) the taps
 // that the clamp folded onto q.  at(p) returns the gradient at sample p.
+// (Tap indices are compile-time constants: the coefficients are kernel
+// arguments, which a run-time index would have to fetch from memory.)
 template <class F>
 __device__ __forceinline__ float adjoint_tap(const LossCoef& L, int q, int n, F at) {
   float acc = 0.0f;
-  const int lo = max(q - kLR, 0), hi = min(q + kLR, n - 1);
-  for (int p = lo; p <= hi; ++p) acc += at(p) * L.k[q - p + kLR];
-  if (q == 0)
-    for (int p = 0; p <= min(kLR - 1, n - 1); ++p) acc += at(p) * L.kcum[kLR - 1 - p];
-  if (q == n - 1)
-    for (int p = max(n - kLR, 0); p <= n - 1; ++p) acc += at(p) * L.kdown[n - p + kLR];
+#pragma unroll
+  for (int t = 0; t < 11; ++t) {  // sample p = q + 5 - t reaches q through tap t
+    const int p = q + kLR - t;
+    if (p >= 0 && p < n) acc += at(p) * L.k[t];
+  }
+  if (q == 0) {  // taps of samples 0..4 that fell below the image
+#pragma unroll
+    for (int p = 0; p < kLR; ++p)
+      if (p < n) acc += at(p) * L.kcum[kLR - 1 - p];
+  }
+  if (q == n - 1) {  // taps of samples n-1..n-5 that fell beyond it
+#pragma unroll
+    for (int j = 0; j < kLR; ++j) {
+      const int p = n - 1 - j;
+      if (p >= 0) acc += at(p) * L.kdown[kLR + 1 + j];
+    }
+  }
   return acc;
 }
 
@@ -256,12 +297,17 @@ void make_loss_coef(const float* w6, int residual, int degenerate_nan, LossCoef*
 
 uint32_t loss_blocks(uint32_t W, uint32_t H) { return ((W + kLB - 1) / kLB) * ((H + kLB - 1) / kLB); }
 
+// loss_parts: kLossParts floats per view of the batch (all zero on entry;
+// left zero), indexed by global view id like loss_sums.
 void launch_loss(hipStream_t s, const ViewDev* views, int v0, int nv, uint32_t max_blocks, const LossCoef& L,
-                 float* loss_sums) {
+                 float* loss_sums, float* loss_parts) {
   if (nv <= 0 || max_blocks == 0) return;
   const dim3 grid(max_blocks, (uint32_t)nv);
-  k_loss_fwd<<<grid, 256, 0, s>>>(views, v0, L, loss_sums);
+  k_loss_fwd<<<grid, 256, 0, s>>>(views, v0, L, loss_parts);
+  k_loss_finish<<<(uint32_t)nv, kLossParts, 0, s>>>(loss_parts, v0, loss_sums);
   k_loss_bwd<<<grid, 256, 0, s>>>(views, v0, L);
 }
+
+size_t loss_parts_floats(int views) { return (size_t)views * kLossParts; }
 
 }  // namespace rgs

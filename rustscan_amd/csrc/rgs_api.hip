@@ -63,7 +63,8 @@ hipError_t noalloc_setup(int device);
 void make_loss_coef(const float* w6, int residual, int degenerate_nan, LossCoef* L);
 uint32_t loss_blocks(uint32_t W, uint32_t H);
 void launch_loss(hipStream_t s, const ViewDev* views, int v0, int nv, uint32_t max_blocks, const LossCoef& L,
-                 float* loss_sums);
+                 float* loss_sums, float* loss_parts);
+size_t loss_parts_floats(int views);
 int noalloc_sort_pairs(int device, hipStream_t stream, const uint32_t* keys, const uint32_t* values,
                        uint32_t* out_keys, uint32_t* out_values, uint32_t n, const char** err);
 }  // namespace rgs
@@ -108,6 +109,7 @@ struct Workspace {
   std::vector<DevBuf> own_img;  // images kept internal when the caller passes none
   // combined loss: per view, the SSIM partial-derivative maps and the image gradient
   std::vector<DevBuf> loss_grad, loss_vout;
+  DevBuf loss_parts;  // partial loss sums per view (kept zero between uses)
 };
 
 struct ViewSet {
@@ -406,6 +408,10 @@ int check_loss_cfg(rgs_ctx* c, const rgs_loss_cfg& cfg) {
 // image gradient the compositing backward reads.
 int attach_loss_buffers(rgs_ctx* c, ViewSet& vs) {
   Workspace& w = c->ws;
+  if (!w.loss_parts.p) {
+    if (ensure(c, w.loss_parts, loss_parts_floats(kMaxViews) * 4)) return RGS_ERR_HIP;
+    launch_fill_f32(c->stream, w.loss_parts.as<float>(), 0.0f, loss_parts_floats(kMaxViews));
+  }
   if (w.loss_grad.size() < vs.v.size()) w.loss_grad.resize(vs.v.size());
   if (w.loss_vout.size() < vs.v.size()) w.loss_vout.resize(vs.v.size());
   for (size_t i = 0; i < vs.v.size(); ++i) {
@@ -604,7 +610,8 @@ int enqueue_step(rgs_ctx* c, rgs_splats* s, const ViewSet& vs, const rgs_step_cf
     if (!combined) return;
     uint32_t blocks = 0;
     for (int v = v0; v < v0 + cnt; ++v) blocks = std::max(blocks, loss_blocks(vs.v[v].W, vs.v[v].H));
-    launch_loss(c->stream, c->ws.views.as<ViewDev>(), v0, cnt, blocks, lcoef, c->ws.fscal.as<float>() + kFLossBase);
+    launch_loss(c->stream, c->ws.views.as<ViewDev>(), v0, cnt, blocks, lcoef, c->ws.fscal.as<float>() + kFLossBase,
+                c->ws.loss_parts.as<float>());
   };
   launch_step_begin(c->stream, c->ws.counters.as<uint32_t>(), c->ws.fscal.as<float>(), nv);
   launch_fill_f32(c->stream, s->visible.as<float>(), 0.0f, s->npad);
@@ -790,6 +797,7 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   for (auto& b : w.own_img) release(b);
   for (auto& b : w.loss_grad) release(b);
   for (auto& b : w.loss_vout) release(b);
+  release(w.loss_parts);
   release(c->loss_view);
   hipHostFree(c->h_counters);
   hipHostFree(c->h_fscal);
@@ -1141,6 +1149,8 @@ int rgs_l1_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint32_
 int rgs_image_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint32_t W, uint32_t H, float weight,
                         const rgs_loss_cfg* cfg, float* v_out, float* loss_out) {
   if (!c || !pred || !target || !cfg || W == 0 || H == 0 || W > 16384 || H > 16384) return RGS_ERR_INVALID;
+  if ((reinterpret_cast<uintptr_t>(pred) & 15u) != 0)
+    return fail(c, RGS_ERR_INVALID, "pred must be 16-byte aligned (RGBA float rows)");
   HIPCHK(c, hipSetDevice(c->device));
   int rc = check_loss_cfg(c, *cfg);
   if (rc) return rc;
@@ -1148,14 +1158,16 @@ int rgs_image_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint
   const bool plain = !lc.combined;
   if (plain) lc = rgs_loss_cfg{1, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};  // plain L1 through the same kernels
   const size_t np = (size_t)W * H;
-  DevBuf grad, vtmp, acc;
+  DevBuf grad, vtmp, acc, parts;
   if (ensure(c, grad, np * 9 * 4) || ensure(c, acc, 4) || (!v_out && ensure(c, vtmp, np * 16)) ||
-      ensure(c, c->loss_view, sizeof(ViewDev))) {
+      ensure(c, parts, loss_parts_floats(1) * 4) || ensure(c, c->loss_view, sizeof(ViewDev))) {
     release(grad);
     release(vtmp);
     release(acc);
+    release(parts);
     return RGS_ERR_HIP;
   }
+  launch_fill_f32(c->stream, parts.as<float>(), 0.0f, loss_parts_floats(1));
   ViewDev V{};
   V.W = W;
   V.H = H;
@@ -1168,7 +1180,7 @@ int rgs_image_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint
   launch_fill_f32(c->stream, acc.as<float>(), 0.0f, 1);
   if (e == hipSuccess) {
     launch_loss(c->stream, c->loss_view.as<ViewDev>(), 0, 1, loss_blocks(W, H), loss_coef(lc, !plain),
-                acc.as<float>());
+                acc.as<float>(), parts.as<float>());
     float sum = 0.0f;
     e = hipMemcpyAsync(&sum, acc.p, 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1177,6 +1189,7 @@ int rgs_image_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint
   release(grad);
   release(vtmp);
   release(acc);
+  release(parts);
   if (e != hipSuccess) return fail(c, RGS_ERR_HIP, "loss: %s", hipGetErrorString(e));
   return RGS_OK;
 }

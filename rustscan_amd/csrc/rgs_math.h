@@ -135,3 +135,53 @@ RGS_HD float rgs_sigmoid(float x) { return 1.0f / (1.0f + rgs_exp(-x)); }
 
 // 1 / sqrt(x) as two correctly rounded operations.
 RGS_HD float rgs_rsqrt(float x) { return 1.0f / rgs_sqrt(x); }
+
+// ---- topology step: a counter-based random stream and cos(2*pi*u) --------
+// SplitMix64 (Steele, Lea and Flood, 2014): the output for state x.
+RGS_HD uint64_t rgs_splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Uniform float in [0, 1) with 24 random bits: draw `counter` of the stream
+// `seed` (stream and counter are mixed so that streams do not overlap).
+RGS_HD float rgs_uniform01(uint64_t seed, uint64_t counter) {
+  const uint64_t z = rgs_splitmix64(rgs_splitmix64(seed) ^ (counter * 0xd6e8feb86659fd93ull));
+  return (float)(uint32_t)(z >> 40) * 5.9604644775390625e-08f;  // 2^-24
+}
+
+// cos(2*pi*u): reduction to |t| <= 1/2 (t = u - rint(u)), then to an angle
+// y in [0, pi/4] by the quadrant identities cos(pi - x) = -cos(x) and
+// cos(pi/2 - x) = sin(x), and the Taylor series of cos (to x^10) or sin (to
+// x^11) on |y| <= pi/4 (truncation below 1.1e-10).
+RGS_HD float rgs_cos_2pi(float u) {
+  float t = u - rgs_rint(u);
+  if (t < 0.0f) t = -t;                       // [0, 1/2]
+  float sign = 1.0f;
+  if (t > 0.25f) {
+    t = 0.5f - t;                             // cos(pi - x) = -cos(x)
+    sign = -1.0f;
+  }
+  const float y = t * 6.28318530717958648f;   // [0, pi/2]
+  if (y <= 0.785398163397448310f) {
+    const float y2 = y * y;
+    float p = -2.7557319223985893e-07f;       // -1/10!
+    p = rgs_fma(p, y2, 2.4801587301587302e-05f);   // 1/8!
+    p = rgs_fma(p, y2, -1.3888888888888889e-03f);  // -1/6!
+    p = rgs_fma(p, y2, 4.1666666666666667e-02f);   // 1/4!
+    p = rgs_fma(p, y2, -0.5f);
+    p = rgs_fma(p, y2, 1.0f);
+    return sign * p;
+  }
+  const float z = 1.57079632679489662f - y;   // cos(y) = sin(pi/2 - y)
+  const float z2 = z * z;
+  float p = -2.5052108385441720e-08f;         // -1/11!
+  p = rgs_fma(p, z2, 2.7557319223985891e-06f);   // 1/9!
+  p = rgs_fma(p, z2, -1.9841269841269841e-04f);  // -1/7!
+  p = rgs_fma(p, z2, 8.3333333333333333e-03f);   // 1/5!
+  p = rgs_fma(p, z2, -1.6666666666666667e-01f);  // -1/3!
+  p = rgs_fma(p, z2 * z, z);
+  return sign * p;
+}
