@@ -171,7 +171,77 @@ int rgs_adam_reset(rgs_ctx* ctx, rgs_splats* s);
  * visible_observations, actual_visible_observations.  Synchronous.
  * rgs_densify_reset zeroes them (reset_accumulators, trainer.rs:861-871). */
 int rgs_densify_export(rgs_ctx* ctx, const rgs_splats* s, float* out);
+/* Sets the accumulators from [10][N] (same order), e.g. to resume a run. */
+int rgs_densify_import(rgs_ctx* ctx, rgs_splats* s, const float* in);
 int rgs_densify_reset(rgs_ctx* ctx, rgs_splats* s);
+/* ---- topology step: densify / prune / opacity reset ---------------------
+ * WgpuTrainer::apply_topology_mutations (engine/trainer.rs:792-859) with its
+ * planning (topology/bridge.rs:25-172, topology/mod.rs:203-981,
+ * topology/schedule.rs:38-133) and rebuild (topology/apply.rs:10-188) on the
+ * device: candidate analysis from the densification accumulators, weighted
+ * sampling of the sources to refine, survivors compacted, refined pairs
+ * appended, refine decay and opacity reset, the per-Gaussian birth iteration
+ * and invisible-window state remapped, the accumulators cleared, and the Adam
+ * state cleared when the splats changed (trainer.rs:851-853).  The splat
+ * handle keeps its identity; its count may change (rgs_splats_count).
+ * Differences from the reference: its random draws (rand 0.8.5's StdRng and
+ * sample_weighted) are replaced by SplitMix64 draws with the same algorithms
+ * (Efraimidis-Spirakis keys, largest first; Box-Muller), and the scene centre
+ * is summed in double.  Synchronous. */
+typedef struct rgs_topology_cfg {
+  uint32_t iterations;                  /* TrainingConfig::iterations (30000) */
+  uint32_t max_initial_gaussians;       /* growth budget floor (100000) */
+  uint64_t frame_shuffle_seed;          /* (0) */
+  uint32_t refine_every;                /* (160) */
+  uint32_t opacity_reset_interval;      /* epochs (10) */
+  uint32_t opacity_reset_mode;          /* 0 decay (default), 1 reset */
+  int32_t topology_freeze_after_epoch;  /* -1: none */
+  int32_t growth_freeze_after_epoch;    /* -1: none */
+  int32_t prune_until_epoch;            /* -1: none */
+  uint32_t split_score_mode;            /* 0 baseline, 1 abs, 2 abs-pixel, 3 abs-pixel-depth */
+  uint32_t growth_stop_iter;            /* (15000) */
+  float growth_grad_threshold;          /* (0.00014) */
+  float split_grad_threshold;           /* (0.00014) */
+  float depth_scale_gamma;              /* (0.37) */
+  float growth_select_fraction;         /* (0.25) */
+  float opacity_decay, scale_decay;     /* refine decay (0, 0) */
+  uint32_t prune_mode;                  /* 0 threshold, 1 weight (default), 2 visibility-weight */
+  uint32_t prune_min_age;               /* iterations (5) */
+  uint32_t prune_invisible_epochs;      /* (10) */
+  uint32_t prune_visibility_dry_run;    /* diagnostics only (0) */
+  float prune_opacity_threshold;        /* (1/255) */
+  float prune_visibility_threshold;     /* (0.05) */
+  float prune_high_opacity_threshold;   /* (0.8) */
+} rgs_topology_cfg;
+
+typedef struct rgs_topology_schedule {
+  int32_t completed_epoch; /* -1: none */
+  int32_t densify, prune, reset_opacity, allow_extra_growth;
+} rgs_topology_schedule;
+
+typedef struct rgs_topology_result {
+  rgs_topology_schedule schedule;
+  int64_t before, after, added, pruned;
+  int64_t clone_candidates, split_candidates, growth_candidates, prune_candidates;
+  int64_t replacement_count, extra_growth_count, selected_count;
+  int32_t mutated;    /* the splats were rebuilt (plan.mutates_splats) */
+  int32_t adam_reset; /* the optimizer state was cleared */
+} rgs_topology_result;
+
+/* should_apply_topology_step (schedule.rs:97-112): 1 when this iteration's
+ * schedule densifies, prunes or resets opacities; rgs_topology_get_schedule
+ * returns the schedule itself (schedule_topology, schedule.rs:38-95). */
+int rgs_topology_should_apply(const rgs_topology_cfg* cfg, uint32_t iteration, uint32_t frame_count);
+int rgs_topology_get_schedule(const rgs_topology_cfg* cfg, uint32_t iteration, uint32_t frame_count,
+                              rgs_topology_schedule* out);
+int rgs_topology_step(rgs_ctx* ctx, rgs_splats* s, const rgs_topology_cfg* cfg, uint32_t iteration,
+                      uint32_t frame_count, rgs_topology_result* result);
+/* The per-Gaussian topology state (trainer.rs:76-77, 171-172): birth
+ * iteration and invisible windows, [N] each; either pointer may be NULL.
+ * Synchronous.  Both start at 0 when splats are created or imported. */
+int rgs_topology_export_state(rgs_ctx* ctx, const rgs_splats* s, uint32_t* birth, uint32_t* invisible_windows);
+int rgs_topology_import_state(rgs_ctx* ctx, rgs_splats* s, const uint32_t* birth, const uint32_t* invisible_windows);
+
 /* AdamState::step of the three groups (optimizer.rs:27-32, 46-51):
  * steps[0] transforms, steps[1] SH coefficients, steps[2] opacities.  Read,
  * or set (to resume from a checkpoint); synchronous. */

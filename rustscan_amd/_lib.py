@@ -95,6 +95,50 @@ class RgsStepCfg(ctypes.Structure):
     ]
 
 
+class RgsTopologyCfg(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_uint32),
+        ("max_initial_gaussians", c_uint32),
+        ("frame_shuffle_seed", ctypes.c_uint64),
+        ("refine_every", c_uint32),
+        ("opacity_reset_interval", c_uint32),
+        ("opacity_reset_mode", c_uint32),
+        ("topology_freeze_after_epoch", ctypes.c_int32),
+        ("growth_freeze_after_epoch", ctypes.c_int32),
+        ("prune_until_epoch", ctypes.c_int32),
+        ("split_score_mode", c_uint32),
+        ("growth_stop_iter", c_uint32),
+        ("growth_grad_threshold", c_float),
+        ("split_grad_threshold", c_float),
+        ("depth_scale_gamma", c_float),
+        ("growth_select_fraction", c_float),
+        ("opacity_decay", c_float),
+        ("scale_decay", c_float),
+        ("prune_mode", c_uint32),
+        ("prune_min_age", c_uint32),
+        ("prune_invisible_epochs", c_uint32),
+        ("prune_visibility_dry_run", c_uint32),
+        ("prune_opacity_threshold", c_float),
+        ("prune_visibility_threshold", c_float),
+        ("prune_high_opacity_threshold", c_float),
+    ]
+
+
+class RgsTopologySchedule(ctypes.Structure):
+    _fields_ = [("completed_epoch", ctypes.c_int32), ("densify", ctypes.c_int32), ("prune", ctypes.c_int32),
+                ("reset_opacity", ctypes.c_int32), ("allow_extra_growth", ctypes.c_int32)]
+
+
+class RgsTopologyResult(ctypes.Structure):
+    _fields_ = [
+        ("schedule", RgsTopologySchedule),
+        ("before", c_int64), ("after", c_int64), ("added", c_int64), ("pruned", c_int64),
+        ("clone_candidates", c_int64), ("split_candidates", c_int64), ("growth_candidates", c_int64),
+        ("prune_candidates", c_int64), ("replacement_count", c_int64), ("extra_growth_count", c_int64),
+        ("selected_count", c_int64), ("mutated", ctypes.c_int32), ("adam_reset", ctypes.c_int32),
+    ]
+
+
 class RgsRasterCfg(ctypes.Structure):
     _fields_ = [("tile_width", c_uint32), ("batch_depth", c_uint32), ("persistent", c_uint32)]
 
@@ -130,6 +174,13 @@ _SIGNATURES = {
     "rgs_adam_reset": (c_int, [c_void_p, c_void_p]),
     "rgs_densify_export": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_densify_reset": (c_int, [c_void_p, c_void_p]),
+    "rgs_densify_import": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rgs_topology_should_apply": (c_int, [POINTER(RgsTopologyCfg), c_uint32, c_uint32]),
+    "rgs_topology_get_schedule": (c_int, [POINTER(RgsTopologyCfg), c_uint32, c_uint32, POINTER(RgsTopologySchedule)]),
+    "rgs_topology_step": (c_int, [c_void_p, c_void_p, POINTER(RgsTopologyCfg), c_uint32, c_uint32,
+                                  POINTER(RgsTopologyResult)]),
+    "rgs_topology_export_state": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rgs_topology_import_state": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "rgs_adam_get_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_set_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_export_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),

@@ -16,6 +16,7 @@
 
 #include "../../include/rgs.h"
 #include "rgs_internal.h"
+#include "rgs_host.h"
 
 namespace rgs {
 void launch_project_fwd(int deg, hipStream_t s, const float* P, uint32_t npad, uint32_t n, const ViewDev* views,
@@ -72,188 +73,6 @@ int noalloc_sort_pairs(int device, hipStream_t stream, const uint32_t* keys, con
 using namespace rgs;
 
 namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
-uint32_t ceil_log2(uint64_t x) {
-  uint32_t b = 0;
-  while ((1ull << b) < x) ++b;
-  return b;
-}
-
-}  // namespace
-
-struct rgs_splats {
-  int64_t n = 0;
-  uint32_t npad = 0;
-  int deg = 0, nk = 1, nplanes = 11;
-  DevBuf P, G, M, V, stats, visible, lr;
-  // densification accumulators [10][npad] (trainer.rs:66-75) and one step's
-  // increments [5][npad] (see k_densify_commit)
-  DevBuf dens, dinc;
-  DevBuf step;  // Adam step counters of the three groups (AdamState::step, optimizer.rs:27-32)
-};
-
-struct Workspace {
-  uint32_t n = 0;
-  int view_slots = 0;
-  size_t cap_inst = 0;   // n * view_slots
-  size_t cap_isect = 0;  // intersection capacity
-  uint32_t tiles_cap = 0;
-  DevBuf views, proj, keys, vals, cnt, scnt, sk1, sv1, sk2, sv2, offs, ik0, iv0, ik1, iv1, ranges, acc, acc_dbg, counters,
-      fscal, scratch, visi;
-  std::vector<DevBuf> own_img;  // images kept internal when the caller passes none
-  // combined loss: per view, the SSIM partial-derivative maps and the image gradient
-  std::vector<DevBuf> loss_grad, loss_vout;
-  DevBuf loss_parts;  // partial loss sums per view (kept zero between uses)
-};
-
-struct ViewSet {
-  std::vector<ViewDev> v;
-  uint32_t tiles_total = 0;
-  int tw = 16;
-};
-
-struct StepPlanKey {
-  int64_t n;
-  int deg, nviews, mode, tw, batch, persistent, densify;
-  std::vector<uint32_t> dims;
-  std::vector<uint32_t> loss;  // bits of the loss configuration (captured as kernel arguments)
-  bool operator<(const StepPlanKey& o) const {
-    return std::tie(n, deg, nviews, mode, tw, batch, persistent, densify, dims, loss) <
-           std::tie(o.n, o.deg, o.nviews, o.mode, o.tw, o.batch, o.persistent, o.densify, o.dims, o.loss);
-  }
-  bool operator==(const StepPlanKey& o) const { return !(*this < o) && !(o < *this); }
-};
-
-struct rgs_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int num_cus = 256;
-  rgs_raster_cfg raster{16, 256, 1};
-  Workspace ws;
-  // last forward state
-  bool fwd_valid = false;
-  const rgs_splats* fwd_splats = nullptr;
-  int fwd_v0 = 0, fwd_nv = 0;
-  uint32_t* fwd_order = nullptr;
-  uint32_t* fwd_isect_keys = nullptr;
-  uint32_t* fwd_isect_vals = nullptr;
-  ViewSet fwd_views;
-  float fwd_blur = 0.0f;
-  bool bwd_done = false;
-  bool keep_raster_grads = false;
-  // graph
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  bool graph_valid = false;
-  StepPlanKey gkey{};
-  const rgs_splats* gsplats = nullptr;
-  std::map<StepPlanKey, bool> sized;
-  // batched mode: view groups [starts[i], starts[i+1]) launched together, so
-  // that no launch holds more than launch_limit intersections
-  std::map<StepPlanKey, std::vector<int>> groups;
-  uint64_t launch_limit = 1600000000ull;
-  // host mirrors
-  uint32_t* h_counters = nullptr;
-  float* h_fscal = nullptr;
-  // counters block as the last completed training step left it: every step
-  // ends with a device-to-host copy into this pinned block (a node of the
-  // captured graph too), read by the next call after step_done
-  uint32_t* h_step = nullptr;
-  hipEvent_t step_done = nullptr;
-  bool step_pending = false;
-  std::map<std::vector<uint32_t>, rgs_raster_cfg> tuned;
-  // the single view of rgs_image_loss_grad (kept apart from the forward state)
-  DevBuf loss_view;
-  // per-stage timing
-  bool timing = false;
-  bool capturing = false;
-  struct Pending {
-    int stage;
-    hipEvent_t a, b;
-  };
-  std::vector<Pending> pending;
-  double stage_ms[RGS_STAGE_COUNT] = {};
-  int64_t stage_calls[RGS_STAGE_COUNT] = {};
-  // fragment counters (device, kFragCounters x u64) when enabled
-  DevBuf frag;
-  bool count_frags = false;
-  bool plain_sorts = false;  // debug: unpacked tile sort, count gather in the depth sort
-  bool no_cull = false;      // debug: no per-block culling in the compositing kernels
-};
-
-namespace {
-
-int fail(rgs_ctx* c, int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  if (c) c->err = buf;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(ctx, RGS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-int ensure(rgs_ctx* c, DevBuf& b, size_t bytes) {
-  if (b.bytes >= bytes && b.p) return RGS_OK;
-  if (b.p) hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
-  if (bytes == 0) return RGS_OK;
-  hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess) {
-    b.p = nullptr;
-    return fail(c, RGS_ERR_HIP, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-  }
-  b.bytes = bytes;
-  return RGS_OK;
-}
-
-void release(DevBuf& b) {
-  if (b.p) hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
-}
-
-// Records HIP events around one stage when timing is on (never while capturing).
-struct StageTimer {
-  rgs_ctx* c;
-  int stage;
-  hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(rgs_ctx* ctx, int st) : c(ctx), stage(st) {
-    if (c->timing && !c->capturing) {
-      hipEventCreate(&a);
-      hipEventCreate(&b);
-      hipEventRecord(a, c->stream);
-    }
-  }
-  ~StageTimer() {
-    if (a) {
-      hipEventRecord(b, c->stream);
-      c->pending.push_back({stage, a, b});
-    }
-  }
-};
-
-void invalidate_graph(rgs_ctx* c) {
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
-  c->graph_valid = false;
-}
 
 int check_camera(rgs_ctx* c, const rgs_camera& cam) {
   if (cam.width == 0 || cam.height == 0 || cam.width > 16384 || cam.height > 16384)
@@ -870,6 +689,8 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   rc |= ensure(c, s->step, 4 * kAdamGroups);
   rc |= ensure(c, s->dens, (size_t)s->npad * 4 * 10);
   rc |= ensure(c, s->dinc, (size_t)s->npad * 4 * 5);
+  rc |= ensure(c, s->birth, (size_t)s->npad * 4);
+  rc |= ensure(c, s->invis, (size_t)s->npad * 4);
   if (rc) {
     rgs_splats_destroy(c, s);
     return RGS_ERR_HIP;
@@ -885,6 +706,8 @@ int rgs_splats_create(rgs_ctx* c, int64_t n, int deg, rgs_splats** out) {
   launch_fill_u32(c->stream, s->step.as<uint32_t>(), 0u, kAdamGroups);
   launch_fill_f32(c->stream, s->dens.as<float>(), 0.0f, (size_t)s->npad * 10);
   launch_fill_f32(c->stream, s->dinc.as<float>(), 0.0f, (size_t)s->npad * 5);
+  launch_fill_u32(c->stream, s->birth.as<uint32_t>(), 0u, s->npad);
+  launch_fill_u32(c->stream, s->invis.as<uint32_t>(), 0u, s->npad);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rgs_splats_destroy(c, s);
     return fail(c, RGS_ERR_HIP, "stream error");
@@ -900,7 +723,8 @@ void rgs_splats_destroy(rgs_ctx* c, rgs_splats* s) {
     if (c->gsplats == s) invalidate_graph(c);
     if (c->fwd_splats == s) c->fwd_valid = false;
   }
-  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr, &s->step, &s->dens, &s->dinc};
+  DevBuf* bufs[] = {&s->P, &s->G, &s->M, &s->V, &s->stats, &s->visible, &s->lr, &s->step, &s->dens, &s->dinc,
+                    &s->birth, &s->invis};
   for (DevBuf* b : bufs) release(*b);
   delete s;
 }
@@ -952,6 +776,9 @@ int rgs_splats_import_aos(rgs_ctx* c, rgs_splats* s, const float* tf, const floa
   release(tmp);
   if (rc) return rc;
   if (c->fwd_splats == s) c->fwd_valid = false;
+  // new splats: their topology state starts over (trainer.rs:171-172)
+  launch_fill_u32(c->stream, s->birth.as<uint32_t>(), 0u, s->npad);
+  launch_fill_u32(c->stream, s->invis.as<uint32_t>(), 0u, s->npad);
   return rgs_adam_reset(c, s);
 }
 
@@ -1043,6 +870,15 @@ int rgs_densify_export(rgs_ctx* c, const rgs_splats* s, float* out) {
   if (!c || !s || !out) return RGS_ERR_INVALID;
   for (int k = 0; k < 10; ++k)
     HIPCHK(c, hipMemcpyAsync(out + (size_t)k * s->n, s->dens.as<float>() + (size_t)k * s->npad, 4 * (size_t)s->n,
+                             hipMemcpyDefault, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RGS_OK;
+}
+
+int rgs_densify_import(rgs_ctx* c, rgs_splats* s, const float* in) {
+  if (!c || !s || !in) return RGS_ERR_INVALID;
+  for (int k = 0; k < 10; ++k)
+    HIPCHK(c, hipMemcpyAsync(s->dens.as<float>() + (size_t)k * s->npad, in + (size_t)k * s->n, 4 * (size_t)s->n,
                              hipMemcpyDefault, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RGS_OK;

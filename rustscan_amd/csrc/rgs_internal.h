@@ -80,6 +80,36 @@ __host__ __device__ inline int adam_group_of_plane(int plane) { return plane < 1
 
 constexpr int kAcc = 16;
 
+// Per-step parameters of the topology kernels (k_topology.hip), host-computed.
+struct TopoParams {
+  uint32_t n, npad, nsh;  // Gaussians, plane stride, SH planes (3K)
+  uint32_t iteration;
+  uint32_t densify, prune, reset_opacity;
+  uint32_t split_score_mode, prune_mode, prune_min_age, prune_invisible_epochs;
+  float growth_grad_threshold, split_grad_threshold, depth_scale_gamma;
+  float prune_opacity_threshold, prune_visibility_threshold, prune_high_opacity_threshold;
+  uint64_t seed;
+};
+
+// Parameters of the rebuild kernel k_topo_apply.
+struct TopoApply {
+  uint32_t nn, old_npad, new_npad, nplanes, iteration;
+  uint32_t decay, reset_opacity;
+  float scale_decay, opacity_decay, train_t;
+};
+
+// counters of the topology analysis and sampling kernels
+enum TopoCount : int {
+  kTopoPrune = 0,
+  kTopoGrowth = 1,
+  kTopoClone = 2,
+  kTopoSplit = 3,
+  kTopoReplacementSources = 4,
+  kTopoEligible = 5,  // eligible sources of the last sampling pass
+  kTopoCountNum = 8,
+};
+
+
 // Coefficients of the combined loss kernels (k_loss.hip), passed by value.
 struct LossCoef {
   float l1, ssim, grad;  // term weights

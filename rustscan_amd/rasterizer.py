@@ -192,8 +192,37 @@ class DeviceSplats:
         self.ctx.check(self.ctx.lib.rgs_densify_export(self.ctx.h, self.h, out.ctypes.data))
         return {name: out[i] for i, name in enumerate(L.DENSIFY_FIELDS)}
 
+    def set_densify_stats(self, stats: np.ndarray):
+        """Sets the accumulators from a [10, N] array (rgs_densify_export order)."""
+        a = np.ascontiguousarray(stats, np.float32).reshape(10, self.n)
+        self.ctx.check(self.ctx.lib.rgs_densify_import(self.ctx.h, self.h, a.ctypes.data))
+
     def reset_densify_stats(self):
         self.ctx.check(self.ctx.lib.rgs_densify_reset(self.ctx.h, self.h))
+
+    def topology_state(self):
+        """(birth iteration, invisible windows) per Gaussian (trainer.rs:76-77)."""
+        birth = np.zeros(self.n, np.uint32)
+        win = np.zeros(self.n, np.uint32)
+        self.ctx.check(self.ctx.lib.rgs_topology_export_state(self.ctx.h, self.h, birth.ctypes.data, win.ctypes.data))
+        return birth, win
+
+    def set_topology_state(self, birth, invisible_windows):
+        b = np.ascontiguousarray(birth, np.uint32)
+        w = np.ascontiguousarray(invisible_windows, np.uint32)
+        self.ctx.check(self.ctx.lib.rgs_topology_import_state(self.ctx.h, self.h, b.ctypes.data, w.ctypes.data))
+
+    def topology_step(self, cfg: "TopologyConfig", iteration: int, frame_count: int) -> dict:
+        """apply_topology_mutations (engine/trainer.rs:792-859) on the device;
+        the store may change size."""
+        res = L.RgsTopologyResult()
+        c = cfg.to_cfg()
+        self.ctx.check(self.ctx.lib.rgs_topology_step(self.ctx.h, self.h, ctypes.byref(c), int(iteration),
+                                                      int(frame_count), ctypes.byref(res)))
+        self.n = int(res.after)
+        out = {f: getattr(res, f) for f, _ in L.RgsTopologyResult._fields_ if f != "schedule"}
+        out["schedule"] = {f: getattr(res.schedule, f) for f, _ in L.RgsTopologySchedule._fields_}
+        return out
 
     def visible(self) -> np.ndarray:
         out = np.zeros(self.n, np.float32)
@@ -241,6 +270,50 @@ def l1_loss_grad(ctx: Context, pred, target, weight: float = 1.0):
     ctx.check(ctx.lib.rgs_l1_loss_grad(ctx.h, pred.data_ptr(), target.contiguous().data_ptr(), w, h, float(weight),
                                        v_out.data_ptr(), ctypes.byref(loss)))
     return loss.value, v_out
+
+
+@dataclass
+class TopologyConfig:
+    """The TrainingConfig fields the topology step reads (training/config.rs:
+    262-383 LiteGS topology, growth, refine and pruning groups; 666-686
+    max_initial_gaussians; 712-721 frame_shuffle_seed; 790-809 iterations),
+    with RustGS's defaults.  Enumerations as integers: opacity_reset_mode 0
+    decay / 1 reset; split_score_mode 0 baseline / 1 abs / 2 abs-pixel / 3
+    abs-pixel-depth; prune_mode 0 threshold / 1 weight / 2 visibility-weight;
+    optional epochs -1 for none."""
+
+    iterations: int = 30000
+    max_initial_gaussians: int = 100000
+    frame_shuffle_seed: int = 0
+    refine_every: int = 160
+    opacity_reset_interval: int = 10
+    opacity_reset_mode: int = 0
+    topology_freeze_after_epoch: int = -1
+    growth_freeze_after_epoch: int = -1
+    prune_until_epoch: int = -1
+    split_score_mode: int = 0
+    growth_stop_iter: int = 15000
+    growth_grad_threshold: float = 0.00014
+    split_grad_threshold: float = 0.00014
+    depth_scale_gamma: float = 0.37
+    growth_select_fraction: float = 0.25
+    opacity_decay: float = 0.0
+    scale_decay: float = 0.0
+    prune_mode: int = 1
+    prune_min_age: int = 5
+    prune_invisible_epochs: int = 10
+    prune_visibility_dry_run: int = 0
+    prune_opacity_threshold: float = 1.0 / 255.0
+    prune_visibility_threshold: float = 0.05
+    prune_high_opacity_threshold: float = 0.8
+
+    def to_cfg(self) -> L.RgsTopologyCfg:
+        return L.RgsTopologyCfg(**{f: getattr(self, f) for f, _ in L.RgsTopologyCfg._fields_})
+
+    def should_apply(self, iteration: int, frame_count: int) -> bool:
+        """should_apply_topology_step (topology/schedule.rs:97-112)."""
+        c = self.to_cfg()
+        return bool(L.lib().rgs_topology_should_apply(ctypes.byref(c), int(iteration), int(frame_count)))
 
 
 @dataclass
