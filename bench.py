@@ -82,6 +82,39 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a))
 
 
+def run_eval_render(ctx, state, cam, target, reps: int = 10) -> dict:
+    """GaussianRenderer::render_splats / TiledRenderer semantics on the device
+    for one view of the scene, and one evaluation frame (render + metrics):
+    milliseconds per synchronous call over `reps` calls after one warm-up."""
+    import torch
+
+    from rustscan_amd.evaluation import GaussianRenderer, TiledRenderer, evaluate_frame
+    from rustscan_amd.rasterizer import DeviceSplats
+
+    ds = DeviceSplats(ctx, state)
+    gr = GaussianRenderer(cam.width, cam.height).with_raster_cov_blur(0.3)
+    gr.render_splats(ctx, ds, cam)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gr.render_splats(ctx, ds, cam)
+    t_render = (time.perf_counter() - t0) / reps
+    evaluate_frame(ctx, ds, cam, target, 0.3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        metrics, _ = evaluate_frame(ctx, ds, cam, target, 0.3)
+    t_eval = (time.perf_counter() - t0) / reps
+    color, depth = TiledRenderer(cam.width, cam.height).with_raster_cov_blur(0.3).render_camera_splats(ctx, ds, cam)
+    images = (color.cpu().numpy(), depth.cpu().numpy())
+    ds.close()
+    px = cam.width * cam.height
+    log(f"eval render {cam.width}x{cam.height}: {1e3 * t_render:.2f} ms/frame; evaluate frame "
+        f"{1e3 * t_eval:.2f} ms (psnr {metrics['psnr_db']:.3f} dB)")
+    return {"view": f"{cam.width}x{cam.height}", "render_ms": round(1e3 * t_render, 3),
+            "render_mpix_per_s": round(px / t_render / 1e6, 1), "evaluate_frame_ms": round(1e3 * t_eval, 3),
+            "psnr_db": metrics["psnr_db"], "_images": images}
+
+
 def run_config3(ctx, opt, init, gt, steps: int = 200) -> dict:
     """The reference's training semantics: one 1280x720 view and one Adam
     update per step, cycling over 32 ring cameras, targets rendered from the
@@ -406,6 +439,11 @@ def main() -> int:
     mpix["pixels_per_step"] = pixels
     log(f"GPU Mpix/s: {mpix}")
 
+    # ---- evaluation render on the device (the reference's CPU renderer
+    # semantics, GaussianRenderer::render_splats) and one evaluation frame
+    # (render + PSNR and sharpness), wall clock per synchronous call
+    eval_render = run_eval_render(ctx, state, cams[2], targets[2])
+
     # ---- the reference's CPU renderer (render/tiled_renderer.rs restated, single-threaded)
     cpu_render = None
     if not args.no_cpu_baseline:
@@ -413,12 +451,25 @@ def main() -> int:
 
         cam = cams[2]  # 1280x720
         t0 = time.perf_counter()
-        O.tiled_renderer(state, cam, cov_blur=0.3)
+        ref_color, ref_depth = O.tiled_renderer(state, cam, cov_blur=0.3)
         t_r = time.perf_counter() - t0
         cpu_render = {"value": round(cam.width * cam.height / t_r / 1e6, 4), "unit": "Mpix/s", "cores": 1,
                       "kind": "port", "sample": f"one {cam.width}x{cam.height} forward render of the 1,048,576-"
                                                 "Gaussian scene by the CPU tiled renderer"}
         log(f"cpu tiled renderer: {t_r:.1f}s for one {cam.width}x{cam.height} view")
+        if eval_render is not None:
+            # the device render against the reference arithmetic (libm exp): equal up to exp's ulps
+            gc, gd = eval_render.pop("_images")
+            eval_render["vs_cpu_renderer"] = {
+                "max_abs_color": float(np.abs(gc - ref_color).max()),
+                "color_values_differing": int((gc != ref_color).sum()),
+                "depth_pixels_differing": int((gd != ref_depth).sum()),
+                "pixels": int(cam.width * cam.height),
+            }
+            eval_render["gpu_over_cpu_renderer"] = round(eval_render["render_mpix_per_s"] / cpu_render["value"], 1)
+            log(f"eval render vs cpu renderer: {eval_render['vs_cpu_renderer']}")
+    if eval_render is not None:
+        eval_render.pop("_images", None)
 
     # ---- CPU baseline: the oracle for one full step, and gradient agreement with it
     cpu = None
@@ -504,6 +555,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_tiled_renderer": cpu_render,
+        "evaluation_render": eval_render,
         "config3": None if args.no_config3 else run_config3(ctx, opt, init, gt),
         "config5": None if args.no_config5 else run_config5(args, ctx, opt),
         "device": dev_name,

@@ -257,6 +257,45 @@ int rgs_adam_import_moments(rgs_ctx* ctx, rgs_splats* s, const float* m_transfor
                             const float* m_raw_opac, const float* v_transforms, const float* v_sh,
                             const float* v_raw_opac);
 
+/* ---- evaluation render: the reference's CPU renderer on the device -------
+ * TiledRenderer::render_camera_splats (render/tiled_renderer.rs:174-492):
+ * colour from sh.rs evaluate_sh_rgb (degrees 0-4), projection without EWA
+ * clip, 16x16 tile lists from compute_tile_bounds, per-tile stable depth
+ * order, alpha accumulation to 0.999, background, clamp to [0, 1].
+ * color [H][W][3] floats, depth [H][W] (f32::MAX where no splat reached
+ * alpha 0.01); device pointers, either may be NULL.  background: 3 host
+ * floats or NULL (black).  cov_blur: raster_cov_blur (negative or NaN -> 0).
+ * Image sides 1..16384.  Synchronous.  Independent of the training forward
+ * state. */
+int rgs_tiled_render(rgs_ctx* ctx, const rgs_splats* s, const rgs_camera* cam, const float* background,
+                     float cov_blur, float* color, float* depth);
+/* GaussianRenderer::render_splats (render/renderer.rs:72-98): the same render
+ * with colour quantised to `(clamp(v, 0, 1) * 255) as u8` ([H][W][3] bytes)
+ * and depth f32::MAX mapped to 0. */
+int rgs_render_splats(rgs_ctx* ctx, const rgs_splats* s, const rgs_camera* cam, const float* background,
+                      float cov_blur, uint8_t* color, float* depth);
+
+/* Per-frame evaluation numbers (evaluation/core.rs:442-449). */
+typedef struct rgs_frame_metrics {
+  float psnr_db;                                   /* compute_psnr_f32, core.rs:238-256 */
+  float render_grad, target_grad;                  /* compute_gradient_sharpness_f32, :258-280 */
+  float render_lap, target_lap;                    /* compute_laplacian_sharpness_f32, :282-306 */
+  float sharpness_grad_ratio, sharpness_lap_ratio; /* sharpness_ratio, :313-319 */
+} rgs_frame_metrics;
+/* The metrics of a rendered image against a target, both [H][W][3] floats
+ * on the device.  The per-pixel terms are the reference's float32 terms;
+ * their sums are float64 (the reference sums in float32, left to right).
+ * Synchronous. */
+int rgs_image_metrics(rgs_ctx* ctx, const float* rendered, const float* target, uint32_t width, uint32_t height,
+                      rgs_frame_metrics* out);
+/* One frame of evaluate_splats (evaluation/core.rs:431-449): render with
+ * SplatEvaluationRenderer (GaussianRenderer, black background, colour as
+ * u8 / 255, core.rs:362-377) and the metrics against target ([H][W][3]
+ * floats, device).  color: [H][W][3] bytes (device) receiving the render, or
+ * NULL.  Synchronous. */
+int rgs_evaluate_frame(rgs_ctx* ctx, const rgs_splats* s, const rgs_camera* cam, float cov_blur,
+                       const float* target, uint8_t* color, rgs_frame_metrics* out);
+
 /* ---- forward: render_forward (forward/mod.rs:156-294) --------------------
  * Projection + SH (ProjectionBackend, projection.rs:44-50; ProjectVisibleBackend,
  * project_visible.rs:36-45), depth sort (SortingBackend, sorting.rs:8-61), tile

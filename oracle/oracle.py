@@ -82,7 +82,8 @@ def lib():
         L.orc_projection_terms_f64.argtypes = [c_int64, c_void_p, c_void_p, POINTER(OrcCamera), c_double, c_void_p,
                                                c_void_p]
         L.orc_tiled_renderer.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                         POINTER(OrcCamera), c_void_p, c_float, c_void_p, c_void_p]
+                                         POINTER(OrcCamera), c_void_p, c_float, c_int, c_void_p, c_void_p]
+        L.orc_tr_sh_rgb.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p]
         L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_exp_core.argtypes = [c_void_p, c_void_p, c_int64]
@@ -276,8 +277,11 @@ def projection_terms_f64(tf64, op64, cam, cov_blur=0.3):
     return vd, comp
 
 
-def tiled_renderer(splats, cam, background=(0.0, 0.0, 0.0), cov_blur=0.3):
-    """The reference's CPU renderer (render/tiled_renderer.rs), single-threaded."""
+def tiled_renderer(splats, cam, background=(0.0, 0.0, 0.0), cov_blur=0.3, gpu_math=False):
+    """The reference's CPU renderer (TiledRenderer::render_camera_splats,
+    render/tiled_renderer.rs:174-492), single-threaded: (color [H,W,3] in
+    [0, 1], depth [H,W] with f32::MAX where no splat reached alpha 0.01).
+    gpu_math=True swaps libm's expf for the library's rgs_exp."""
     tf = _f32(splats.transforms)
     pos = np.ascontiguousarray(tf[:, 0:3])
     rot = np.ascontiguousarray(tf[:, 3:7])
@@ -289,9 +293,29 @@ def tiled_renderer(splats, cam, background=(0.0, 0.0, 0.0), cov_blur=0.3):
     depth = np.zeros((cam.height, cam.width), np.float32)
     bg = (c_float * 3)(*background)
     lib().orc_tiled_renderer(splats.n, splats.sh_degree, pos.ctypes.data, ls.ctypes.data, rot.ctypes.data,
-                             op.ctypes.data, sh.ctypes.data, ctypes.byref(c), bg, float(cov_blur), color.ctypes.data,
-                             depth.ctypes.data)
+                             op.ctypes.data, sh.ctypes.data, ctypes.byref(c), bg, float(cov_blur), int(gpu_math),
+                             color.ctypes.data, depth.ctypes.data)
     return color, depth
+
+
+def gaussian_renderer(splats, cam, background=(0.0, 0.0, 0.0), cov_blur=0.3, gpu_math=False):
+    """GaussianRenderer::render_splats (render/renderer.rs:72-98): the tiled
+    render with colour quantised as `(clamp(v, 0, 1) * 255) as u8` (NaN -> 0)
+    and depth f32::MAX mapped to 0."""
+    color, depth = tiled_renderer(splats, cam, background, cov_blur, gpu_math)
+    v = np.clip(color, np.float32(0), np.float32(1)) * np.float32(255.0)
+    c8 = np.where(np.isnan(v), 0, np.trunc(v)).astype(np.uint8)
+    depth = np.where(depth == np.float32(3.4028235e38), np.float32(0), depth)
+    return c8, depth
+
+
+def tr_sh_rgb(sh, degree, dirs):
+    """sh.rs:35-112 evaluate_sh_rgb: sh [N,K,3], dirs [N,3] -> rgb [N,3]."""
+    sh = _f32(sh)
+    dirs = _f32(dirs)
+    out = np.zeros((dirs.shape[0], 3), np.float32)
+    lib().orc_tr_sh_rgb(dirs.shape[0], degree, sh.ctypes.data, dirs.ctypes.data, out.ctypes.data)
+    return out
 
 
 def math_exp(x: np.ndarray) -> np.ndarray:

@@ -1300,17 +1300,86 @@ void orc_projection_terms_f64(int64_t n, const double* tf, const double* op, con
   delete p;
 }
 
+// sh.rs:35-112 evaluate_sh_rgb with its f32 literals (degrees 0-4; missing
+// coefficients read as zero, sh.rs:114-121).
+static void tr_sh_rgb(const float* c, int nk, int degree, const float vd[3], float col[3]) {
+  auto co = [&](int k, int ch) { return k < nk ? c[k * 3 + ch] : 0.0f; };
+  for (int ch = 0; ch < 3; ++ch) col[ch] = 0.2820948f * co(0, ch);
+  auto add = [&](int k, float s) {
+    for (int ch = 0; ch < 3; ++ch) col[ch] += co(k, ch) * s;
+  };
+  const float x = vd[0], y = vd[1], z = vd[2];
+  if (degree >= 1) {
+    add(1, 0.48860252f * -y);
+    add(2, 0.48860252f * z);
+    add(3, 0.48860252f * -x);
+  }
+  if (degree >= 2) {
+    const float z2 = z * z, f0b = -1.0925485f * z, f1a = 0.54627424f, fc1 = x * x - y * y, fs1 = 2.0f * x * y;
+    add(4, f1a * fs1);
+    add(5, f0b * y);
+    add(6, 0.9461747f * z2 - 0.31539157f);
+    add(7, f0b * x);
+    add(8, f1a * fc1);
+    if (degree >= 3) {
+      const float f0c = -2.285229f * z2 + 0.4570458f, f1b = 1.4453057f * z, f2a = -0.5900436f;
+      const float fc2 = x * fc1 - y * fs1, fs2 = x * fs1 + y * fc1;
+      add(9, f2a * fs2);
+      add(10, f1b * fs1);
+      add(11, f0c * y);
+      add(12, z * (1.8658817f * z2 - 1.119529f));
+      add(13, f0c * x);
+      add(14, f1b * fc1);
+      add(15, f2a * fc2);
+      if (degree >= 4) {
+        const float f0d = z * (-4.683326f * z2 + 2.0071397f), f1c = 3.3116114f * z2 - 0.47308734f;
+        const float f2b = -1.7701308f * z, f3a = 0.6258357f;
+        const float fc3 = x * fc2 - y * fs2, fs3 = x * fs2 + y * fc2;
+        const float p12 = z * (1.8658817f * z2 - 1.119529f), p6 = 0.9461747f * z2 - 0.31539157f;
+        add(16, f3a * fs3);
+        add(17, f2b * fs2);
+        add(18, f1c * fs1);
+        add(19, f0d * y);
+        add(20, 1.9843135f * z * p12 - 1.0062306f * p6);
+        add(21, f0d * x);
+        add(22, f1c * fc1);
+        add(23, f2b * fc2);
+        add(24, f3a * fc3);
+      }
+    }
+  }
+  for (int ch = 0; ch < 3; ++ch) col[ch] += 0.5f;
+}
+
+void orc_tr_sh_rgb(int64_t n, int degree, const float* sh, const float* dirs, float* rgb) {
+  const int nk = (degree + 1) * (degree + 1);
+  for (int64_t i = 0; i < n; ++i) tr_sh_rgb(sh + i * nk * 3, nk, degree, dirs + i * 3, rgb + i * 3);
+}
+
+// Rust's `f as usize` for the non-negative clamp results of
+// compute_tile_bounds: truncation, NaN and negatives to 0, saturating above.
+static uint64_t tr_as_usize(float f) {
+  if (!(f > 0.0f)) return 0;
+  if (f >= 18446744073709551616.0f) return UINT64_MAX;
+  return (uint64_t)f;
+}
+
 void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const float* log_scales,
                         const float* rotations, const float* opacity_logits, const float* sh, const orc_camera* cam,
-                        const float* background, float cov_blur, float* color, float* depth_out) {
-  // render/tiled_renderer.rs:174-492 and sh.rs:35-112 (evaluate_sh_rgb), single-threaded
+                        const float* background, float cov_blur, int gpu_math, float* color, float* depth_out) {
+  // render/tiled_renderer.rs:174-492 (render_camera_splats, project_projected_gaussian,
+  // compute_tile_bounds, render_projected_with_background), single-threaded.
+  // gpu_math = 0: libm expf, as the reference; 1: rgs_exp, the library's
+  // exponential (the two differ by a few ulp at most).
+  // Rust's f32::max / f32::min return the non-NaN operand: fmaxf / fminf.
+  auto EXP = [gpu_math](float v) { return gpu_math ? rgs_exp(v) : std::exp(v); };
   struct PG {
     float x, y, depth, cxx, cxy, cyy, opacity, col[3];
   };
   const uint32_t W = cam->width, H = cam->height;
-  const int tw = 16;
-  const uint32_t ntx = (W + tw - 1) / tw, nty = (H + tw - 1) / tw;
-  const float blur = cov_blur > 0.0f ? cov_blur : 0.0f;
+  const uint64_t tw = 16;
+  const uint64_t ntx = (W + tw - 1) / tw, nty = (H + tw - 1) / tw;
+  const float blur = std::fmax(cov_blur, 0.0f);  // with_raster_cov_blur (tiled_renderer.rs:68-71)
   float r[3][3], tr3[3];
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) r[i][j] = cam->viewmat[j * 4 + i];  // rows
@@ -1328,55 +1397,28 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
     } else {
       for (int i = 0; i < 3; ++i) vd[i] /= len;
     }
-    // sh.rs evaluate_sh_rgb with its f32 literals
-    const float* c = sh + idx * nk * 3;
     float col[3];
-    for (int ch = 0; ch < 3; ++ch) col[ch] = 0.2820948f * c[ch];
-    auto add = [&](int k, float s) {
-      if (k < nk)
-        for (int ch = 0; ch < 3; ++ch) col[ch] += c[k * 3 + ch] * s;
-    };
-    const float x = vd[0], y = vd[1], z = vd[2];
-    if (sh_degree >= 1) {
-      add(1, 0.48860252f * -y);
-      add(2, 0.48860252f * z);
-      add(3, 0.48860252f * -x);
-    }
-    if (sh_degree >= 2) {
-      const float z2 = z * z, f0b = -1.0925485f * z, f1a = 0.54627424f, fc1 = x * x - y * y, fs1 = 2.0f * x * y;
-      add(4, f1a * fs1);
-      add(5, f0b * y);
-      add(6, 0.9461747f * z2 - 0.31539157f);
-      add(7, f0b * x);
-      add(8, f1a * fc1);
-      if (sh_degree >= 3) {
-        const float f0c = -2.285229f * z2 + 0.4570458f, f1b = 1.4453057f * z, f2a = -0.5900436f;
-        const float fc2 = x * fc1 - y * fs1, fs2 = x * fs1 + y * fc1;
-        add(9, f2a * fs2);
-        add(10, f1b * fs1);
-        add(11, f0c * y);
-        add(12, z * (1.8658817f * z2 - 1.119529f));
-        add(13, f0c * x);
-        add(14, f1b * fc1);
-        add(15, f2a * fc2);
-      }
-    }
-    for (int ch = 0; ch < 3; ++ch) col[ch] += 0.5f;
+    tr_sh_rgb(sh + idx * nk * 3, nk, sh_degree, vd, col);
     // project_projected_gaussian (tiled_renderer.rs:389-492)
     const float wx = pos[0], wy = pos[1], wz = pos[2];
     const float ccx = r[0][0] * wx + r[0][1] * wy + r[0][2] * wz + tr3[0];
     const float ccy = r[1][0] * wx + r[1][1] * wy + r[1][2] * wz + tr3[1];
     const float ccz = r[2][0] * wx + r[2][1] * wy + r[2][2] * wz + tr3[2];
     if (ccz <= 0.0f) continue;
+    // A NaN depth passes the test above, but such a splat's centre is NaN, so
+    // it never contributes; kept in a tile list it would make the depth
+    // comparison a non-total order, under which the reference's sort_by
+    // leaves the order unspecified (and may panic).  It is left out.
+    if (std::isnan(ccz)) continue;
     const float ppx = cam->fx * ccx / ccz + cam->cx, ppy = cam->fy * ccy / ccz + cam->cy;
     const float* q = rotations + idx * 4;
     const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
     const float r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
     const float r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
     const float r20 = 2 * (qx * qz - qw * qy), r21 = 2 * (qy * qz + qw * qx), r22 = 1 - 2 * (qx * qx + qy * qy);
-    const float sx = std::fabs(std::fabs(std::exp(log_scales[idx * 3]))),
-                sy = std::fabs(std::fabs(std::exp(log_scales[idx * 3 + 1]))),
-                sz = std::fabs(std::fabs(std::exp(log_scales[idx * 3 + 2])));
+    const float sx = std::fabs(std::fabs(EXP(log_scales[idx * 3]))),
+                sy = std::fabs(std::fabs(EXP(log_scales[idx * 3 + 1]))),
+                sz = std::fabs(std::fabs(EXP(log_scales[idx * 3 + 2])));
     const float sxx = sx * sx, syy = sy * sy, szz = sz * sz;
     const float cw00 = r00 * r00 * sxx + r01 * r01 * syy + r02 * r02 * szz;
     const float cw01 = r00 * r10 * sxx + r01 * r11 * syy + r02 * r12 * szz;
@@ -1407,9 +1449,9 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
     const float det_raw = cxx * cyy - cxy * cxy;
     cxx += blur;
     cyy += blur;
-    const float det_bl = std::max(cxx * cyy - cxy * cxy, 1e-12f);
+    const float det_bl = std::fmax(cxx * cyy - cxy * cxy, 1e-12f);
     if (det_raw <= 0.0f) continue;
-    const float opacity = 1.0f / (1.0f + std::exp(-opacity_logits[idx])) * std::sqrt(det_raw / det_bl);
+    const float opacity = 1.0f / (1.0f + EXP(-opacity_logits[idx])) * std::sqrt(det_raw / det_bl);
     if (opacity < 1.0f / 255.0f) continue;
     pr.push_back({ppx, ppy, ccz, cxx, cxy, cyy, opacity, {col[0], col[1], col[2]}});
   }
@@ -1417,24 +1459,25 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
   std::fill(color, color + (size_t)W * H * 3, 0.0f);
   std::fill(depth_out, depth_out + (size_t)W * H, 3.4028235e38f);
   std::vector<std::vector<uint32_t>> lists((size_t)ntx * nty);
-  for (uint32_t idx = 0; idx < pr.size(); ++idx) {
+  for (uint32_t idx = 0; idx < pr.size(); ++idx) {  // compute_tile_bounds with tile_alpha = 4
     const PG& g = pr[idx];
-    const float sgx = std::max(std::sqrt(g.cxx * 4.0f), 1.0f), sgy = std::max(std::sqrt(g.cyy * 4.0f), 1.0f);
-    auto to_us = [](float f) -> size_t { return f <= 0.0f ? 0 : (size_t)f; };
-    const size_t xmin = to_us(std::max(g.x - 3.0f * sgx, 0.0f));
-    const size_t xmax = to_us(std::min(g.x + 3.0f * sgx, (float)W - 1.0f));
-    const size_t ymin = to_us(std::max(g.y - 3.0f * sgy, 0.0f));
-    const size_t ymax = to_us(std::min(g.y + 3.0f * sgy, (float)H - 1.0f));
-    const size_t txmin = xmin / tw, txmax = (xmax + tw - 1) / tw, tymin = ymin / tw, tymax = (ymax + tw - 1) / tw;
-    for (size_t ty = tymin; ty < tymax; ++ty)
-      for (size_t tx = txmin; tx < txmax; ++tx) lists[ty * ntx + tx].push_back(idx);
+    const float sgx = std::fmax(std::sqrt(g.cxx * 4.0f), 1.0f), sgy = std::fmax(std::sqrt(g.cyy * 4.0f), 1.0f);
+    const uint64_t xmin = tr_as_usize(std::fmax(g.x - 3.0f * sgx, 0.0f));
+    const uint64_t xmax = tr_as_usize(std::fmin(g.x + 3.0f * sgx, (float)W - 1.0f));
+    const uint64_t ymin = tr_as_usize(std::fmax(g.y - 3.0f * sgy, 0.0f));
+    const uint64_t ymax = tr_as_usize(std::fmin(g.y + 3.0f * sgy, (float)H - 1.0f));
+    const uint64_t txmin = xmin / tw, txmax = (xmax + tw - 1) / tw, tymin = ymin / tw, tymax = (ymax + tw - 1) / tw;
+    for (uint64_t ty = tymin; ty < tymax; ++ty)
+      for (uint64_t tx = txmin; tx < txmax; ++tx) lists[ty * ntx + tx].push_back(idx);
   }
-  for (uint32_t ty = 0; ty < nty; ++ty)
-    for (uint32_t tx = 0; tx < ntx; ++tx) {
-      auto& L = lists[(size_t)ty * ntx + tx];
+  for (uint64_t ty = 0; ty < nty; ++ty)
+    for (uint64_t tx = 0; tx < ntx; ++tx) {
+      auto& L = lists[ty * ntx + tx];
       if (L.empty()) continue;
+      // stable sort by depth; a NaN depth compares equal (partial_cmp fallback)
       std::stable_sort(L.begin(), L.end(), [&](uint32_t a, uint32_t b) { return pr[a].depth < pr[b].depth; });
-      const uint32_t x0 = tx * tw, y0 = ty * tw, x1 = std::min(x0 + tw, W), y1 = std::min(y0 + tw, H);
+      const uint32_t x0 = (uint32_t)(tx * tw), y0 = (uint32_t)(ty * tw);
+      const uint32_t x1 = std::min<uint32_t>(x0 + (uint32_t)tw, W), y1 = std::min<uint32_t>(y0 + (uint32_t)tw, H);
       for (uint32_t gi : L) {
         const PG& g = pr[gi];
         const float det = g.cxx * g.cyy - g.cxy * g.cxy;
@@ -1447,7 +1490,7 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
             const float dx = g.x - ((float)px + 0.5f), dy = g.y - ((float)py + 0.5f);
             const float d2 = (g.cyy * dx * dx - 2.0f * g.cxy * dx * dy + g.cxx * dy * dy) * inv;
             if (d2 < 9.0f) {
-              const float w = std::exp(-0.5f * d2) * g.opacity;
+              const float w = EXP(-0.5f * d2) * g.opacity;
               if (w > 0.001f) {
                 const float a = w * (1.0f - alpha_buf[id]);
                 if (a > 0.0f) {
@@ -1465,7 +1508,11 @@ void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions, const 
     const float t = 1.0f - alpha_buf[id];
     for (int ch = 0; ch < 3; ++ch) color[id * 3 + ch] += (bg ? bg[ch] : 0.0f) * t;
   }
-  for (size_t i = 0; i < (size_t)W * H * 3; ++i) color[i] = std::min(std::max(color[i], 0.0f), 1.0f);
+  // f32::clamp(0, 1): NaN stays NaN
+  for (size_t i = 0; i < (size_t)W * H * 3; ++i) {
+    const float v = color[i];
+    color[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  }
 }
 
 void orc_math_exp(const float* x, float* y, int64_t n) {

@@ -122,12 +122,15 @@ void orc_render_bruteforce(const orc_render* r, float* out_rgba);
 
 /* ---- the reference's CPU renderer (render/tiled_renderer.rs), restated ----
  * positions/log_scales [N*3], rotations wxyz [N*4], opacity_logits [N],
- * sh [N*K*3].  Writes color[H*W*3], depth[H*W]. Single-threaded. */
+ * sh [N*K*3].  Writes color[H*W*3], depth[H*W]. Single-threaded.
+ * gpu_math: 0 = libm expf (the reference's), 1 = rgs_exp. */
 void orc_tiled_renderer(int64_t n, int sh_degree, const float* positions,
                         const float* log_scales, const float* rotations,
                         const float* opacity_logits, const float* sh,
                         const orc_camera* cam, const float* background,
-                        float cov_blur, float* color, float* depth);
+                        float cov_blur, int gpu_math, float* color, float* depth);
+/* sh.rs evaluate_sh_rgb (degrees 0-4): sh [N*K*3], dirs [N*3] -> rgb [N*3] */
+void orc_tr_sh_rgb(int64_t n, int degree, const float* sh, const float* dirs, float* rgb);
 
 /* ---- scalar math exported for pinning tests ---- */
 void orc_math_exp(const float* x, float* y, int64_t n);

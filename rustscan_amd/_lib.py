@@ -147,6 +147,11 @@ class RgsCounts(ctypes.Structure):
     _fields_ = [("num_visible", c_int64), ("num_intersections", c_int64), ("num_tiles", c_int64)]
 
 
+class RgsFrameMetrics(ctypes.Structure):
+    _fields_ = [("psnr_db", c_float), ("render_grad", c_float), ("target_grad", c_float), ("render_lap", c_float),
+                ("target_lap", c_float), ("sharpness_grad_ratio", c_float), ("sharpness_lap_ratio", c_float)]
+
+
 class RgsError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"rgs error {code}: {message}")
@@ -185,6 +190,13 @@ _SIGNATURES = {
     "rgs_adam_set_steps": (c_int, [c_void_p, c_void_p, c_void_p]),
     "rgs_adam_export_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),
     "rgs_adam_import_moments": (c_int, [c_void_p, c_void_p] + [c_void_p] * 6),
+    "rgs_tiled_render": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_float), c_float, c_void_p,
+                                 c_void_p]),
+    "rgs_render_splats": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), POINTER(c_float), c_float, c_void_p,
+                                  c_void_p]),
+    "rgs_image_metrics": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, POINTER(RgsFrameMetrics)]),
+    "rgs_evaluate_frame": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_float, c_void_p, c_void_p,
+                                   POINTER(RgsFrameMetrics)]),
     "rgs_render_forward": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
                                    POINTER(c_void_p)]),
     "rgs_render_backward": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),

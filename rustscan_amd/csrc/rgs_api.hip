@@ -618,6 +618,7 @@ void rgs_ctx_destroy(rgs_ctx* c) {
   for (auto& b : w.loss_vout) release(b);
   release(w.loss_parts);
   release(c->loss_view);
+  c->eval.release_all();
   hipHostFree(c->h_counters);
   hipHostFree(c->h_fscal);
   hipHostFree(c->h_step);

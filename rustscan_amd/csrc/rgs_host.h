@@ -79,6 +79,22 @@ struct StepPlanKey {
   bool operator==(const StepPlanKey& o) const { return !(*this < o) && !(o < *this); }
 };
 
+// Scratch of the evaluation renderer and metrics (grown on demand, kept
+// between calls: evaluation renders one frame per call).
+struct EvalScratch {
+  DevBuf rec, box, live, ntl, dkey, pos, ck, cv, kb, vb, offs, rec_s, box_s, ik0, iv0, ik1, iv1, ranges, sort, scan,
+      counts, parts, sums, img8;
+  void release_all() {
+    DevBuf* b[] = {&rec, &box, &live, &ntl, &dkey, &pos, &ck, &cv, &kb, &vb, &offs, &rec_s, &box_s,
+                   &ik0, &iv0, &ik1, &iv1, &ranges, &sort, &scan, &counts, &parts, &sums, &img8};
+    for (DevBuf* x : b) {
+      if (x->p) hipFree(x->p);
+      x->p = nullptr;
+      x->bytes = 0;
+    }
+  }
+};
+
 struct rgs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -120,6 +136,7 @@ struct rgs_ctx {
   std::map<std::vector<uint32_t>, rgs_raster_cfg> tuned;
   // the single view of rgs_image_loss_grad (kept apart from the forward state)
   DevBuf loss_view;
+  EvalScratch eval;
   // per-stage timing
   bool timing = false;
   bool capturing = false;

@@ -36,6 +36,21 @@ struct ViewDev {
   float loss_weight;
 };
 
+// Camera of the evaluation renderer (TiledRenderer): rotation rows and
+// translation of the world->camera transform, intrinsics, the covariance
+// blur floor and the tile grid (16x16 pixel tiles).
+struct EvalCam {
+  float r[9];  // row-major rows of the rotation
+  float t[3];
+  float pos[3];  // camera centre in world space
+  float fx, fy, cx, cy;
+  float blur;
+  uint32_t W, H, ntx, nty;
+};
+constexpr uint32_t kEvalTile = 16;
+constexpr uint32_t kEvalMaxSide = 16384;  // image side limit (tile ranges pack in 16 bits)
+constexpr int kEvalSums = 8;               // double sums per block of the metrics kernel
+
 // Projected splat instance, 64 bytes, the unit every later stage gathers by
 // instance.
 //   p0 = (x, y, a/2, b)        conic = [[a, b], [b, c]]
