@@ -296,6 +296,43 @@ int rgs_image_metrics(rgs_ctx* ctx, const float* rendered, const float* target, 
 int rgs_evaluate_frame(rgs_ctx* ctx, const rgs_splats* s, const rgs_camera* cam, float cov_blur,
                        const float* target, uint8_t* color, rgs_frame_metrics* out);
 
+/* ---- scene files: io/scene_io.rs -----------------------------------------
+ * SplatMetadata (scene_io.rs:12-19). */
+typedef struct rgs_splat_metadata {
+  uint64_t iterations;
+  float final_loss;
+  uint64_t gaussian_count;
+  uint32_t sh_degree;
+} rgs_splat_metadata;
+typedef struct rgs_scene rgs_scene; /* splats read from a file, in host memory */
+/* save_splats (scene_io.rs:617-627): by extension, .ply (save_splats_ply,
+ * :485-550: ASCII 3DGS layout, f_rest channel-major, floats as Rust prints
+ * them) or .splat (save_splats_splat, :552-615).  HostSplats arrays (host):
+ * positions [N*3], log_scales [N*3], rotations wxyz [N*4], opacity_logits [N],
+ * sh_coeffs [N*K*3] (K = (sh_degree+1)^2, coefficient-major).  meta NULL:
+ * zeros, count N, degree sh_degree.  ctx may be NULL (no device is used);
+ * then rgs_io_last_error() holds the message. */
+int rgs_save_splats(rgs_ctx* ctx, const char* path, int64_t n, int sh_degree, const float* positions,
+                    const float* log_scales, const float* rotations, const float* opacity_logits,
+                    const float* sh_coeffs, const rgs_splat_metadata* meta);
+/* load_splats (scene_io.rs:872-878): .ply (load_splats_ply, :629-757: ASCII or
+ * binary little-endian, raw-logit or probability opacity, channel-major or
+ * interleaved f_rest, SH degree <= 3) or .splat (load_splats_splat,
+ * :759-870).  *out is freed with rgs_scene_free.  ctx may be NULL. */
+int rgs_scene_load(rgs_ctx* ctx, const char* path, rgs_scene** out);
+int rgs_scene_info(const rgs_scene* scene, int64_t* n, int* sh_degree, rgs_splat_metadata* meta);
+/* Copies the HostSplats arrays (layouts as in rgs_save_splats); any pointer may be NULL. */
+int rgs_scene_get(const rgs_scene* scene, float* positions, float* log_scales, float* rotations,
+                  float* opacity_logits, float* sh_coeffs);
+void rgs_scene_free(rgs_scene* scene);
+/* A device splat store from a loaded scene (host_splats_to_device,
+ * engine/splats.rs:26-63); N >= 1. */
+int rgs_scene_to_splats(rgs_ctx* ctx, const rgs_scene* scene, rgs_splats** out);
+/* save_splats of a device store (its current parameters). */
+int rgs_splats_save(rgs_ctx* ctx, const rgs_splats* s, const char* path, const rgs_splat_metadata* meta);
+/* Message of the last failed scene-file call made with ctx == NULL on this thread. */
+const char* rgs_io_last_error(void);
+
 /* ---- forward: render_forward (forward/mod.rs:156-294) --------------------
  * Projection + SH (ProjectionBackend, projection.rs:44-50; ProjectVisibleBackend,
  * project_visible.rs:36-45), depth sort (SortingBackend, sorting.rs:8-61), tile

@@ -152,6 +152,11 @@ class RgsFrameMetrics(ctypes.Structure):
                 ("target_lap", c_float), ("sharpness_grad_ratio", c_float), ("sharpness_lap_ratio", c_float)]
 
 
+class RgsSplatMetadata(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("final_loss", c_float), ("gaussian_count", ctypes.c_uint64),
+                ("sh_degree", c_uint32)]
+
+
 class RgsError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"rgs error {code}: {message}")
@@ -197,6 +202,15 @@ _SIGNATURES = {
     "rgs_image_metrics": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, POINTER(RgsFrameMetrics)]),
     "rgs_evaluate_frame": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_float, c_void_p, c_void_p,
                                    POINTER(RgsFrameMetrics)]),
+    "rgs_save_splats": (c_int, [c_void_p, ctypes.c_char_p, c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, POINTER(RgsSplatMetadata)]),
+    "rgs_scene_load": (c_int, [c_void_p, ctypes.c_char_p, POINTER(c_void_p)]),
+    "rgs_scene_info": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int), POINTER(RgsSplatMetadata)]),
+    "rgs_scene_get": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rgs_scene_free": (None, [c_void_p]),
+    "rgs_scene_to_splats": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
+    "rgs_splats_save": (c_int, [c_void_p, c_void_p, ctypes.c_char_p, POINTER(RgsSplatMetadata)]),
+    "rgs_io_last_error": (ctypes.c_char_p, []),
     "rgs_render_forward": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
                                    POINTER(c_void_p)]),
     "rgs_render_backward": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "librgs_hip.so")
 ORACLE_DIR = os.path.join(REPO_DIR, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "librgs_oracle.so")
 
-SOURCES = ["k_project.hip", "k_bin.hip", "k_raster.hip", "k_misc.hip", "rgs_api.hip", "rgs_noalloc.hip", "k_loss.hip", "k_topology.hip", "rgs_topology.hip", "k_eval.hip", "rgs_eval.hip"]
+SOURCES = ["k_project.hip", "k_bin.hip", "k_raster.hip", "k_misc.hip", "rgs_api.hip", "rgs_noalloc.hip", "k_loss.hip", "k_topology.hip", "rgs_topology.hip", "k_eval.hip", "rgs_eval.hip", "rgs_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RGS_OFFLOAD_ARCH", "gfx950")

@@ -150,6 +150,17 @@ class DeviceSplats:
         self.h = h
         self.upload(host)
 
+    @classmethod
+    def adopt(cls, ctx: Context, handle: c_void_p) -> "DeviceSplats":
+        """Wraps a store the library created (e.g. rgs_scene_to_splats)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = handle
+        self.n = int(ctx.lib.rgs_splats_count(handle))
+        self.sh_degree = int(ctx.lib.rgs_splats_sh_degree(handle))
+        self.k = sh_coeffs(self.sh_degree)
+        return self
+
     def upload(self, host: Splats):
         tf = np.ascontiguousarray(host.transforms, np.float32)
         sh = np.ascontiguousarray(host.sh, np.float32)
