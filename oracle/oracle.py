@@ -78,12 +78,18 @@ def lib():
         L.orc_powi.argtypes = [c_float, ctypes.c_int32]
         L.orc_forward_loss_f64.restype = c_double
         L.orc_forward_loss_f64.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera),
-                                           c_void_p, c_double, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+                                           c_void_p, c_double, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                           c_int64, c_void_p, c_void_p]
+        L.orc_crossing_map_f64.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera),
+                                           c_void_p, c_double, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         L.orc_projection_terms_f64.argtypes = [c_int64, c_void_p, c_void_p, POINTER(OrcCamera), c_double, c_void_p,
                                                c_void_p]
         L.orc_tiled_renderer.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          POINTER(OrcCamera), c_void_p, c_float, c_int, c_void_p, c_void_p]
         L.orc_tr_sh_rgb.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p]
+        L.orc_grad_f64.restype = c_int
+        L.orc_grad_f64.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera), c_void_p,
+                                   c_double, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
         L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_exp_core.argtypes = [c_void_p, c_void_p, c_int64]
@@ -245,9 +251,11 @@ def powi(base: float, e: int) -> float:
 
 
 def forward_loss_f64(splats64, sh_degree, cam, target, cov_blur=0.3, tile_width=16, flags=0, frozen_viewdir=None,
-                     frozen_comp=None, background=(0.0, 0.0, 0.0), image_out=None):
+                     frozen_comp=None, background=(0.0, 0.0, 0.0), image_out=None, frozen_crossing=None):
     """FP64 forward + L1 loss (mean over 3HW) for finite differences;
-    image_out ([H,W,4] float64, optional) receives the FP64 image."""
+    image_out ([H,W,4] float64, optional) receives the FP64 image.
+    frozen_crossing = (gid, pixel mask [H,W] uint8, projection [9]): that
+    splat is composited with the given projection at the masked pixels."""
     tf, sh, op = [np.ascontiguousarray(a, np.float64) for a in splats64]
     n = op.shape[0]
     c = camera(cam)
@@ -258,11 +266,54 @@ def forward_loss_f64(splats64, sh_degree, cam, target, cov_blur=0.3, tile_width=
     if image_out is not None:
         assert image_out.dtype == np.float64 and image_out.flags.c_contiguous
         assert image_out.shape == (cam.height, cam.width, 4)
+    fg, fpix, fproj = -1, None, None
+    if frozen_crossing is not None:
+        fg = int(frozen_crossing[0])
+        fpix = np.ascontiguousarray(frozen_crossing[1], np.uint8)
+        fproj = np.ascontiguousarray(frozen_crossing[2], np.float64)
+        assert fpix.size == cam.height * cam.width and fproj.size == 9
     return lib().orc_forward_loss_f64(n, sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data, ctypes.byref(c),
                                       bg.ctypes.data, float(cov_blur), int(tile_width), tg.ctypes.data, int(flags),
                                       fv.ctypes.data if fv is not None else None,
                                       fc.ctypes.data if fc is not None else None,
-                                      image_out.ctypes.data if image_out is not None else None)
+                                      image_out.ctypes.data if image_out is not None else None, fg,
+                                      fpix.ctypes.data if fpix is not None else None,
+                                      fproj.ctypes.data if fproj is not None else None)
+
+
+def crossing_map_f64(splats64, sh_degree, cam, cov_blur=0.3, tile_width=16, flags=0, frozen_viewdir=None,
+                     frozen_comp=None):
+    """(pixel -> gid of the splat that takes T below 1e-4, -1 for none, as [H,W]
+    int64; projections [N,9] by gid) of the unperturbed FP64 forward."""
+    tf, sh, op = [np.ascontiguousarray(a, np.float64) for a in splats64]
+    n = op.shape[0]
+    c = camera(cam)
+    fv = np.ascontiguousarray(frozen_viewdir, np.float64) if frozen_viewdir is not None else None
+    fc = np.ascontiguousarray(frozen_comp, np.float64) if frozen_comp is not None else None
+    pix = np.zeros((cam.height, cam.width), np.int64)
+    proj = np.zeros((n, 9), np.float64)
+    lib().orc_crossing_map_f64(n, sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data, ctypes.byref(c), None,
+                               float(cov_blur), int(tile_width), int(flags), fv.ctypes.data if fv is not None else None,
+                               fc.ctypes.data if fc is not None else None, pix.ctypes.data, proj.ctypes.data)
+    return pix, proj
+
+
+def grad_f64(splats64, sh_degree, cam, v_out, mode=0, cov_blur=0.3, tile_width=16):
+    """Gradients of the FP64 pipeline taking its own decisions (orc_grad_f64):
+    dict with v_transforms [N,10], v_sh [N,K,3], v_raw_opac [N] (float64)."""
+    tf, sh, op = (np.ascontiguousarray(a, np.float64) for a in splats64)
+    n = tf.shape[0]
+    k = sh.shape[1]
+    c = camera(cam)
+    vo = np.ascontiguousarray(v_out, np.float64)
+    vt = np.zeros((n, 10))
+    vs = np.zeros((n, k, 3))
+    vp = np.zeros(n)
+    rc = lib().orc_grad_f64(n, sh_degree, tf.ctypes.data, sh.ctypes.data, op.ctypes.data, ctypes.byref(c), None,
+                            float(cov_blur), int(tile_width), vo.ctypes.data, int(mode), vt.ctypes.data,
+                            vs.ctypes.data, vp.ctypes.data)
+    assert rc == 0
+    return {"v_transforms": vt, "v_sh": vs, "v_raw_opac": vp}
 
 
 def projection_terms_f64(tf64, op64, cam, cov_blur=0.3):

@@ -494,6 +494,19 @@ template <class T> struct Pipe {
   // taken from the FP32 splats with FP32 arithmetic, the gradient values are
   // computed in T.  Set only on a Pipe<double> built by orc_backward_f64.
   const std::vector<Proj<float>>* shadow = nullptr;
+  // Frozen crossing splat (FP64 finite differences of the reference
+  // gradient): at the pixels marked in frozen_pix, the splat with global id
+  // frozen_g is composited with the projection frozen_proj (its unperturbed
+  // values), so that its parameters do not act there.  The reference's
+  // backward gives a splat no gradient at the pixels where it takes T to
+  // 1e-4 or below (rasterize_backwards.wgsl:108-112), while its gradient for
+  // the earlier splats is that of the forward that includes it
+  // (rasterize.wgsl:70-80).  cross_gid (filled by every forward) records that
+  // splat per pixel, -1 where none.
+  int64_t frozen_g = -1;
+  const uint8_t* frozen_pix = nullptr;
+  Proj<T> frozen_proj{};
+  std::vector<int64_t> cross_gid;
 
   std::vector<uint8_t> pass1;
   std::vector<T> depth;
@@ -715,6 +728,7 @@ template <class T> struct Pipe {
     img.assign((size_t)v.W * v.H * 4, K<T>(0.0));
     visible.assign(n, 0);
     crossing.assign(proj.size(), 0);
+    cross_gid.assign((size_t)v.W * v.H, -1);
     int64_t ev = 0, bl = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : ev, bl)
     for (int64_t t = 0; t < (int64_t)ntiles; ++t) {
@@ -727,8 +741,10 @@ template <class T> struct Pipe {
           const T pc[2] = {(T)px + K<T>(0.5), (T)py + K<T>(0.5)};
           T Tr = K<T>(1.0);
           T acc[3] = {K<T>(0.0), K<T>(0.0), K<T>(0.0)};
+          const size_t pix = (size_t)py * v.W + px;
+          const bool frz = frozen_pix && frozen_pix[pix];
           for (uint32_t i = s; i < e; ++i) {
-            const Proj<T>& sp = proj[isect_cgid[i]];
+            const Proj<T>& sp = frz && (int64_t)order[isect_cgid[i]] == frozen_g ? frozen_proj : proj[isect_cgid[i]];
             ++ev;
             const T dx = sp.x - pc[0], dy = sp.y - pc[1];
             const T sigma = K<T>(0.5) * (sp.a * dx * dx + K<T>(2.0) * sp.b * dx * dy + sp.c * dy * dy);
@@ -745,6 +761,7 @@ template <class T> struct Pipe {
               __atomic_store_n(&visible[g], (uint8_t)1, __ATOMIC_RELAXED);
               if (Tr < K<T>(1e-4)) {
                 __atomic_store_n(&crossing[isect_cgid[i]], (uint8_t)1, __ATOMIC_RELAXED);
+                cross_gid[pix] = (int64_t)g;
                 break;
               }
             }
@@ -1260,11 +1277,18 @@ static Pipe<double>* make_f64(int64_t n, int deg, const double* tf, const double
 
 double orc_forward_loss_f64(int64_t n, int deg, const double* tf, const double* sh, const double* op,
                             const orc_camera* cam, const double* bg, double blur, int tw, const float* target,
-                            int flags, const double* fvd, const double* fcomp, double* out) {
+                            int flags, const double* fvd, const double* fcomp, double* out, int64_t frozen_g,
+                            const uint8_t* frozen_pix, const double* frozen_proj) {
   Pipe<double>* p = make_f64(n, deg, tf, sh, op, cam, bg, blur, tw);
   if (flags & 1) p->frozen_vd = fvd;
   if (flags & 2) p->frozen_comp = fcomp;
   p->bwd_termination = (flags & 4) != 0;
+  if (frozen_pix && frozen_proj && frozen_g >= 0) {
+    p->frozen_g = frozen_g;
+    p->frozen_pix = frozen_pix;
+    const double* f = frozen_proj;
+    p->frozen_proj = Proj<double>{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8]};
+  }
   p->forward();
   const int64_t np = (int64_t)p->v.W * p->v.H;
   double sum = 0.0;
@@ -1273,6 +1297,45 @@ double orc_forward_loss_f64(int64_t n, int deg, const double* tf, const double* 
   if (out) std::copy(p->img.begin(), p->img.end(), out);
   delete p;
   return sum / (double)(3 * np);
+}
+
+// The whole pipeline in FP64 with its own decisions: forward, compositing
+// backward and projection backward.  The gradient of the FP64 function whose
+// finite differences tests take, free of FP32-versus-FP64 threshold
+// mismatches (a splat that the FP32 forward stops at T = 1e-4 and the FP64
+// forward does not).
+int orc_grad_f64(int64_t n, int deg, const double* tf, const double* sh, const double* op, const orc_camera* cam,
+                 const double* bg, double blur, int tw, const double* v_out, int mode, double* v_tf, double* v_sh,
+                 double* v_op) {
+  if (mode != 0 && mode != 1) return 1;
+  Pipe<double>* p = make_f64(n, deg, tf, sh, op, cam, bg, blur, tw);
+  p->forward();
+  std::vector<double> vs, ss;
+  p->raster_bwd(v_out, mode, vs, ss);
+  p->project_bwd(vs, ss, mode, v_tf, v_sh, v_op, nullptr);
+  delete p;
+  return 0;
+}
+
+// The unperturbed FP64 forward's crossing map for finite differences: per
+// pixel the global id of the splat that takes T below 1e-4 (-1: none), and
+// that splat's projection (x, y, a, b, c, r, g, b, opacity) per global id
+// (zeros for culled Gaussians).
+void orc_crossing_map_f64(int64_t n, int deg, const double* tf, const double* sh, const double* op,
+                          const orc_camera* cam, const double* bg, double blur, int tw, int flags, const double* fvd,
+                          const double* fcomp, int64_t* pix_gid, double* proj9) {
+  Pipe<double>* p = make_f64(n, deg, tf, sh, op, cam, bg, blur, tw);
+  if (flags & 1) p->frozen_vd = fvd;
+  if (flags & 2) p->frozen_comp = fcomp;
+  p->forward();
+  std::copy(p->cross_gid.begin(), p->cross_gid.end(), pix_gid);
+  std::fill(proj9, proj9 + n * 9, 0.0);
+  for (size_t c = 0; c < p->proj.size(); ++c) {
+    const Proj<double>& q = p->proj[c];
+    double* o = proj9 + (size_t)p->order[c] * 9;
+    o[0] = q.x; o[1] = q.y; o[2] = q.a; o[3] = q.b; o[4] = q.c; o[5] = q.r; o[6] = q.g; o[7] = q.bl; o[8] = q.o;
+  }
+  delete p;
 }
 
 void orc_projection_terms_f64(int64_t n, const double* tf, const double* op, const orc_camera* cam, double blur,

@@ -105,7 +105,14 @@ double orc_forward_loss_f64(int64_t n, int sh_degree, const double* transforms,
                             const orc_camera* cam, const double* background,
                             double cov_blur, int tile_width, const float* target_rgb,
                             int flags, const double* frozen_viewdir,
-                            const double* frozen_comp, double* out_rgba);
+                            const double* frozen_comp, double* out_rgba,
+                            int64_t frozen_g, const uint8_t* frozen_pix, const double* frozen_proj);
+/* The unperturbed FP64 forward's crossing map: pix_gid [H*W] (global id of
+ * the splat that takes T below 1e-4, -1 for none), proj9 [N*9] (projection
+ * x, y, a, b, c, r, g, b, opacity by global id). */
+void orc_crossing_map_f64(int64_t n, int deg, const double* tf, const double* sh, const double* op,
+                          const orc_camera* cam, const double* bg, double blur, int tw, int flags, const double* fvd,
+                          const double* fcomp, int64_t* pix_gid, double* proj9);
 /* The per-Gaussian view direction and compensation of the FP64 projection,
  * for building the frozen surrogate (NaN rows for culled Gaussians). */
 void orc_projection_terms_f64(int64_t n, const double* transforms,
@@ -119,6 +126,13 @@ void orc_crossing_splats(const orc_render* r, uint8_t* flags);
  * in depth order, same per-pixel arithmetic as K7.  Only valid where the tile
  * test is exact (used on small scenes to check the binning). */
 void orc_render_bruteforce(const orc_render* r, float* out_rgba);
+
+/* The gradient of the FP64 pipeline with its own decisions (forward, raster
+ * and projection backward in double): v_out [H*W*4] double; outputs
+ * v_tf [N*10], v_sh [N*K*3], v_op [N] double. */
+int orc_grad_f64(int64_t n, int deg, const double* tf, const double* sh, const double* op, const orc_camera* cam,
+                 const double* bg, double blur, int tw, const double* v_out, int mode, double* v_tf, double* v_sh,
+                 double* v_op);
 
 /* ---- the reference's CPU renderer (render/tiled_renderer.rs), restated ----
  * positions/log_scales [N*3], rotations wxyz [N*4], opacity_logits [N],

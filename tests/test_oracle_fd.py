@@ -4,15 +4,22 @@ Pins orc_backward in both modes:
   * mode 0 (the reference's gradient, rasterize_backwards.wgsl +
     project_backwards.wgsl) against FD of a surrogate forward that freezes the
     SH view direction and the blur compensation at their unperturbed values
-    and stops compositing before the splat that takes T to <= 1e-4, which is
-    the function whose exact gradient the reference's backward computes
-    (project_backwards.wgsl:68-79,102,110; rasterize_backwards.wgsl:108-112);
+    (project_backwards.wgsl:68-79,102,110) and, at the pixels where the
+    perturbed splat is the one that takes T below 1e-4, composites that splat
+    with its unperturbed projection.  That is the function whose exact
+    gradient the reference's backward computes: it skips the gradient of the
+    splat that ends a pixel (rasterize_backwards.wgsl:108-112) while the
+    gradients of the splats before it are those of the forward that includes
+    it (rasterize.wgsl:70-80, the final colour in v_alpha);
   * mode 1 (the exact gradient) against FD of the unmodified forward.
 FD protocol: FP64 forward, central differences with eps = 1e-4 on 64
 parameters chosen by seed 4 among Gaussians that cover pixels.  A sample is
 rejected when the differences at eps and eps/2 disagree by more than 10%
 (a threshold crossing: alpha = 1/255, T = 1e-4, the 0.999 clamp, a tile
-test or the L1 kink lies inside the stencil); the rejection count is bounded.
+test or the L1 kink lies inside the stencil), or by more than 1% for the
+linear functional, which is smooth elsewhere (in dense scenes small jumps
+from those thresholds are common and stay under 10%); the rejection count
+is bounded.
 Two objectives: a fixed random linear functional of all four image channels
 (smooth apart from the compositing thresholds, so it is held to rel-L2
 1e-3) and the training objective, the L1 loss against a target image (many
@@ -38,9 +45,11 @@ def fd_scene():
     return sc, cam, target
 
 
-def perturbed_loss(orc, s64, sh_degree, cam, target, blur, g, p, delta, flags, fv, fc, weights=None):
+def perturbed_loss(orc, s64, sh_degree, cam, target, blur, g, p, delta, flags, fv, fc, weights=None, cross=None):
     """FP64 objective after adding delta to parameter p of Gaussian g: the L1
-    loss, or sum(weights * image) when weights are given."""
+    loss, or sum(weights * image) when weights are given.  cross: the
+    unperturbed crossing map (crossing_map_f64), whose splat g is frozen at
+    the pixels it ends."""
     tf, sh, op = (a.copy() for a in s64)
     if p < 10:
         tf[g, p] += delta
@@ -50,8 +59,12 @@ def perturbed_loss(orc, s64, sh_degree, cam, target, blur, g, p, delta, flags, f
         q = p - 11
         sh[g, q // 3, q % 3] += delta
     img = np.zeros((cam.height, cam.width, 4), np.float64) if weights is not None else None
+    frozen = None
+    if cross is not None:
+        pix, proj = cross
+        frozen = (g, (pix == g).astype(np.uint8), proj[g])
     loss = orc.forward_loss_f64((tf, sh, op), sh_degree, cam, target, cov_blur=blur, flags=flags,
-                                frozen_viewdir=fv, frozen_comp=fc, image_out=img)
+                                frozen_viewdir=fv, frozen_comp=fc, image_out=img, frozen_crossing=frozen)
     return float((weights * img).sum()) if weights is not None else loss
 
 
@@ -104,7 +117,7 @@ def needle_fd_scene():
 
 
 def run_fd(orc, mode, blur, objective, surrogate=None, make_scene=fd_scene, min_cover=20, params=None,
-           f64_replay=False):
+           f64_replay=False, reject_rel=None):
     """(analytic, FD, rejected) over N_SAMPLES seeded (Gaussian, parameter)
     picks among Gaussians that cover pixels; `params` restricts the parameter
     indices; `f64_replay` takes the analytic gradient from the oracle's FP64
@@ -126,10 +139,13 @@ def run_fd(orc, mode, blur, objective, surrogate=None, make_scene=fd_scene, min_
     k = sc.sh.shape[1]
     if (mode == 0) if surrogate is None else surrogate:
         fv, fc = orc.projection_terms_f64(s64[0], s64[2], cam, cov_blur=blur)
-        flags = 1 | 2 | 4
+        flags = 1 | 2
+        cross = orc.crossing_map_f64(s64, sc.sh_degree, cam, cov_blur=blur, flags=flags, frozen_viewdir=fv,
+                                     frozen_comp=fc)
     else:
         fv = fc = None
         flags = 0
+        cross = None
     rng = scene.Rng(4)
     picks_g = cover[(rng.uniform(N_SAMPLES) * len(cover)).astype(np.int64)]
     if params is None:
@@ -137,13 +153,16 @@ def run_fd(orc, mode, blur, objective, surrogate=None, make_scene=fd_scene, min_
     else:
         params = np.asarray(params)
         picks_p = params[(rng.uniform(N_SAMPLES) * len(params)).astype(np.int64)]
+    if reject_rel is None:
+        reject_rel = 0.01 if objective == "linear" else 0.1
     a_vals, f_vals, rejected = [], [], 0
     for g, p in zip(picks_g, picks_p):
-        L = lambda d: perturbed_loss(orc, s64, sc.sh_degree, cam, target, blur, g, p, d, flags, fv, fc, weights)
+        L = lambda d: perturbed_loss(orc, s64, sc.sh_degree, cam, target, blur, g, p, d, flags, fv, fc, weights,
+                                     cross)
         fd1 = (L(EPS) - L(-EPS)) / (2 * EPS)
         fd2 = (L(EPS / 2) - L(-EPS / 2)) / EPS
         scale = max(abs(fd1), abs(fd2), 1e-12)
-        if abs(fd1 - fd2) > 0.1 * scale:
+        if abs(fd1 - fd2) > reject_rel * scale:
             rejected += 1
             continue
         a_vals.append(analytic(grads, g, p))
