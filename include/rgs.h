@@ -353,6 +353,25 @@ int rgs_render_forward(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, int 
  * rgs_splats_export_grads). */
 int rgs_render_backward(rgs_ctx* ctx, rgs_splats* s, const float* const* v_out);
 
+/* ---- checkpoints: RenderCheckpoint (backward/autodiff.rs:56-71) per call --
+ * The context keeps one forward state for rgs_render_backward.  A checkpoint
+ * holds one more: rgs_render_forward_checkpoint renders like
+ * rgs_render_forward into the checkpoint's own buffers (projected splats,
+ * tile lists, images) and copies the store's parameters as the forward read
+ * them; rgs_render_backward_checkpoint runs the backward of that forward,
+ * with those parameters, into the store's gradients, whatever forwards,
+ * backwards, steps or parameter updates came in between (the store must keep
+ * its count and SH degree).  Each checkpoint holds a workspace of its own,
+ * grown like the context's; destroy checkpoints before their context
+ * (ctx may then be NULL).  Synchronous. */
+typedef struct rgs_checkpoint rgs_checkpoint;
+int rgs_checkpoint_create(rgs_ctx* ctx, rgs_checkpoint** out);
+void rgs_checkpoint_destroy(rgs_ctx* ctx, rgs_checkpoint* ck);
+int rgs_render_forward_checkpoint(rgs_ctx* ctx, rgs_splats* s, const rgs_camera* cams, int n_views,
+                                  const float* background, float cov_blur, float* const* out_rgba,
+                                  rgs_checkpoint* ck);
+int rgs_render_backward_checkpoint(rgs_ctx* ctx, rgs_splats* s, rgs_checkpoint* ck, const float* const* v_out);
+
 /* ---- L1 loss (engine/loss.rs:144-196, trainer.rs:278) ---------------------
  * pred [H][W][4], target [H][W][3] (device); v_out [H][W][4] (device, may be
  * NULL); *loss_out (host) = weight * mean |pred_rgb - target|.  Synchronous

@@ -214,6 +214,11 @@ _SIGNATURES = {
     "rgs_render_forward": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
                                    POINTER(c_void_p)]),
     "rgs_render_backward": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
+    "rgs_checkpoint_create": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "rgs_checkpoint_destroy": (None, [c_void_p, c_void_p]),
+    "rgs_render_forward_checkpoint": (c_int, [c_void_p, c_void_p, POINTER(RgsCamera), c_int, POINTER(c_float), c_float,
+                                              POINTER(c_void_p), c_void_p]),
+    "rgs_render_backward_checkpoint": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
     "rgs_l1_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, c_void_p,
                                  POINTER(c_float)]),
     "rgs_adam_step": (c_int, [c_void_p, c_void_p, POINTER(RgsAdamCfg)]),

@@ -180,7 +180,7 @@ int ensure_workspace(rgs_ctx* c, const rgs_splats* s, int view_slots, uint32_t t
     launch_fill_u32(c->stream, w.counters.as<uint32_t>(), 0u, kCntNum);
     launch_fill_f32(c->stream, w.fscal.as<float>(), 0.0f, kFNum);
   }
-  if (grew) invalidate_graph(c);
+  if (grew && !c->ckpt_active) invalidate_graph(c);
   return RGS_OK;
 }
 
@@ -956,6 +956,83 @@ int rgs_render_backward(rgs_ctx* c, rgs_splats* s, const float* const* v_out) {
   c->bwd_done = true;
   c->fwd_valid = false;  // the rasterizer accumulators were consumed
   return RGS_OK;
+}
+
+// ---- checkpoints: forward states of their own -----------------------------
+namespace {
+void swap_forward_state(rgs_ctx* c, rgs_checkpoint* k) {
+  std::swap(c->ws, k->ws);
+  std::swap(c->fwd_valid, k->fwd_valid);
+  std::swap(c->fwd_splats, k->fwd_splats);
+  std::swap(c->fwd_v0, k->fwd_v0);
+  std::swap(c->fwd_nv, k->fwd_nv);
+  std::swap(c->fwd_order, k->fwd_order);
+  std::swap(c->fwd_isect_keys, k->fwd_isect_keys);
+  std::swap(c->fwd_isect_vals, k->fwd_isect_vals);
+  std::swap(c->fwd_views, k->fwd_views);
+  std::swap(c->fwd_blur, k->fwd_blur);
+  std::swap(c->bwd_done, k->bwd_done);
+  c->ckpt_active = !c->ckpt_active;
+}
+
+void release_workspace(Workspace& w) {
+  DevBuf* bufs[] = {&w.views, &w.proj, &w.keys, &w.vals, &w.cnt, &w.scnt, &w.sk1, &w.sv1, &w.sk2, &w.sv2, &w.offs, &w.ik0,
+                    &w.iv0, &w.ik1, &w.iv1, &w.ranges, &w.acc, &w.acc_dbg, &w.counters, &w.fscal, &w.scratch, &w.visi};
+  for (DevBuf* b : bufs) release(*b);
+  for (auto& b : w.own_img) release(b);
+  for (auto& b : w.loss_grad) release(b);
+  for (auto& b : w.loss_vout) release(b);
+  release(w.loss_parts);
+}
+}  // namespace
+
+int rgs_checkpoint_create(rgs_ctx* c, rgs_checkpoint** out) {
+  if (!c || !out) return RGS_ERR_INVALID;
+  *out = new rgs_checkpoint();
+  return RGS_OK;
+}
+
+void rgs_checkpoint_destroy(rgs_ctx* c, rgs_checkpoint* k) {
+  if (!k) return;
+  if (c) {
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+  }
+  release_workspace(k->ws);
+  release(k->params);
+  delete k;
+}
+
+int rgs_render_forward_checkpoint(rgs_ctx* c, rgs_splats* s, const rgs_camera* cams, int nv, const float* bg,
+                                  float blur, float* const* out_rgba, rgs_checkpoint* k) {
+  if (!c || !s || !cams || !k) return RGS_ERR_INVALID;
+  if (c->ckpt_active) return fail(c, RGS_ERR_STATE, "a checkpoint is already active on this context");
+  swap_forward_state(c, k);
+  int rc = rgs_render_forward(c, s, cams, nv, bg, blur, out_rgba);
+  swap_forward_state(c, k);
+  if (rc) return rc;
+  const size_t bytes = (size_t)s->npad * 4 * s->nplanes;
+  if (ensure(c, k->params, bytes)) return RGS_ERR_HIP;
+  HIPCHK(c, hipMemcpyAsync(k->params.p, s->P.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  k->n = s->n;
+  k->npad = s->npad;
+  k->deg = s->deg;
+  return RGS_OK;
+}
+
+int rgs_render_backward_checkpoint(rgs_ctx* c, rgs_splats* s, rgs_checkpoint* k, const float* const* v_out) {
+  if (!c || !s || !k || !v_out) return RGS_ERR_INVALID;
+  if (c->ckpt_active) return fail(c, RGS_ERR_STATE, "a checkpoint is already active on this context");
+  if (!k->fwd_valid || k->fwd_splats != s || k->n != s->n || k->npad != s->npad || k->deg != s->deg)
+    return fail(c, RGS_ERR_STATE,
+                "rgs_render_backward_checkpoint needs a forward into this checkpoint on these splats (same count)");
+  std::swap(s->P, k->params);  // the parameters the forward read
+  swap_forward_state(c, k);
+  const int rc = rgs_render_backward(c, s, v_out);
+  swap_forward_state(c, k);
+  std::swap(s->P, k->params);
+  return rc;
 }
 
 int rgs_l1_loss_grad(rgs_ctx* c, const float* pred, const float* target, uint32_t W, uint32_t H, float weight,

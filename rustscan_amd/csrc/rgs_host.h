@@ -95,6 +95,28 @@ struct EvalScratch {
   }
 };
 
+// A forward state of its own (RenderCheckpoint, backward/autodiff.rs:56-71):
+// the workspace that holds the projected splats, tile lists and images of one
+// rgs_render_forward_checkpoint call, the forward-state fields, and the
+// parameter planes the forward read.  Swapped into the context for the calls
+// that use it.
+struct rgs_checkpoint {
+  Workspace ws;
+  bool fwd_valid = false;
+  const rgs_splats* fwd_splats = nullptr;
+  int fwd_v0 = 0, fwd_nv = 0;
+  uint32_t* fwd_order = nullptr;
+  uint32_t* fwd_isect_keys = nullptr;
+  uint32_t* fwd_isect_vals = nullptr;
+  ViewSet fwd_views;
+  float fwd_blur = 0.0f;
+  bool bwd_done = false;
+  DevBuf params;
+  int64_t n = 0;
+  uint32_t npad = 0;
+  int deg = -1;
+};
+
 struct rgs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -150,6 +172,7 @@ struct rgs_ctx {
   // fragment counters (device, kFragCounters x u64) when enabled
   DevBuf frag;
   bool count_frags = false;
+  bool ckpt_active = false;  // a checkpoint's workspace is swapped in (the train graph's stays valid)
   bool plain_sorts = false;  // debug: unpacked tile sort, count gather in the depth sort
   bool no_cull = false;      // debug: no per-block culling in the compositing kernels
 };

@@ -252,23 +252,56 @@ class DeviceSplats:
             pass
 
 
-def render_forward(ctx: Context, splats: DeviceSplats, cams, background=(0.0, 0.0, 0.0), cov_blur: float = 0.3):
-    """Renders every camera; returns [H,W,4] float32 torch tensors on the device."""
+class RenderCheckpoint:
+    """A forward state of its own (RenderCheckpoint, backward/autodiff.rs:56-71):
+    pass it to render_forward and later to render_backward; several may be
+    outstanding at once."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = c_void_p()
+        ctx.check(ctx.lib.rgs_checkpoint_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.rgs_checkpoint_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_forward(ctx: Context, splats: DeviceSplats, cams, background=(0.0, 0.0, 0.0), cov_blur: float = 0.3,
+                   checkpoint: RenderCheckpoint | None = None):
+    """Renders every camera; returns [H,W,4] float32 torch tensors on the device.
+    With a checkpoint the forward state goes there instead of the context."""
     torch = _torch()
     arr, nv = _cams(cams)
     outs = [torch.empty((c.height, c.width, 4), dtype=torch.float32, device="cuda") for c in cams]
     bg = (c_float * 3)(*background)
     torch.cuda.synchronize()
-    ctx.check(ctx.lib.rgs_render_forward(ctx.h, splats.h, arr, nv, bg, float(cov_blur), _ptrs(outs)))
+    if checkpoint is None:
+        ctx.check(ctx.lib.rgs_render_forward(ctx.h, splats.h, arr, nv, bg, float(cov_blur), _ptrs(outs)))
+    else:
+        ctx.check(ctx.lib.rgs_render_forward_checkpoint(ctx.h, splats.h, arr, nv, bg, float(cov_blur), _ptrs(outs),
+                                                        checkpoint.h))
     return outs
 
 
-def render_backward(ctx: Context, splats: DeviceSplats, v_outs):
-    """RenderBackward::backward: image gradients -> splat-store gradients."""
+def render_backward(ctx: Context, splats: DeviceSplats, v_outs, checkpoint: RenderCheckpoint | None = None):
+    """RenderBackward::backward: image gradients -> splat-store gradients (of
+    the context's last forward, or of the checkpoint's)."""
     torch = _torch()
     v_outs = [v.contiguous().float() for v in v_outs]
     torch.cuda.synchronize()
-    ctx.check(ctx.lib.rgs_render_backward(ctx.h, splats.h, _ptrs(v_outs)))
+    if checkpoint is None:
+        ctx.check(ctx.lib.rgs_render_backward(ctx.h, splats.h, _ptrs(v_outs)))
+    else:
+        ctx.check(ctx.lib.rgs_render_backward_checkpoint(ctx.h, splats.h, checkpoint.h, _ptrs(v_outs)))
     return splats.grads()
 
 
