@@ -244,13 +244,19 @@ __device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* s_wc, uint32
   return off + below;
 }
 
-// Calls body(s0[j], s1[j], s2[j], j) for the staged splats j < cnt whose bit
+// One staged splat in LDS: three 16-byte words at one address, so the
+// per-splat reads of a wave share a single address register.
+struct alignas(16) StagedSplat {
+  float4 p0, p1, p2;
+};
+
+// Calls body(rec[j].p0, rec[j].p1, rec[j].p2, j) for the staged splats j < cnt whose bit
 // is set in this wave's words, in increasing j, while alive() holds for some
 // lane.  Two splats per trip, the next record fetched one splat ahead into
 // alternating registers.
 template <int BATCH, typename Alive, typename Body>
-__device__ __forceinline__ void for_each_reaching(const uint64_t* words, uint32_t cnt, const float4* s0,
-                                                  const float4* s1, const float2* s2, Alive&& alive, Body&& body) {
+__device__ __forceinline__ void for_each_reaching(const uint64_t* words, uint32_t cnt, const StagedSplat* rec,
+                                                  Alive&& alive, Body&& body) {
   const uint32_t nw = (cnt + 63) / 64;
   for (uint32_t wi = 0; wi < nw; ++wi) {
     const uint64_t wv = words[wi];
@@ -260,22 +266,20 @@ __device__ __forceinline__ void for_each_reaching(const uint64_t* words, uint32_
     if (!wave_any(alive())) return;
     uint32_t ja = wi * 64 + (uint32_t)__builtin_ctzll(m);
     m &= m - 1;
-    float4 p0 = s0[ja], p1 = s1[ja];
-    float2 p2 = s2[ja];
+    float4 p0 = rec[ja].p0, p1 = rec[ja].p1, p2 = rec[ja].p2;
     while (true) {
       const bool hb = m != 0;
       const uint32_t jb = hb ? wi * 64 + (uint32_t)__builtin_ctzll(m) : ja;
       m &= m - 1;
-      const float4 q0 = s0[jb], q1 = s1[jb];
-      const float2 q2 = s2[jb];
+      const float4 q0 = rec[jb].p0, q1 = rec[jb].p1, q2 = rec[jb].p2;
       body(p0, p1, p2, ja);
       if (!hb || !wave_any(alive())) break;
       const bool ha = m != 0;
       ja = ha ? wi * 64 + (uint32_t)__builtin_ctzll(m) : jb;
       m &= m - 1;
-      p0 = s0[ja];
-      p1 = s1[ja];
-      p2 = s2[ja];
+      p0 = rec[ja].p0;
+      p1 = rec[ja].p1;
+      p2 = rec[ja].p2;
       body(q0, q1, q2, jb);
       if (!ha || !wave_any(alive())) break;
     }
@@ -285,11 +289,10 @@ __device__ __forceinline__ void for_each_reaching(const uint64_t* words, uint32_
 template <int TW, int BATCH, bool MULTI, bool COUNT>
 __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A) {
   constexpr int RW = RasterShape<TW>::RW, SUB = RasterShape<TW>::SUB, NT = RasterShape<TW>::NT;
-  __shared__ float4 s0[BATCH];  // x, y, a/2, b
-  __shared__ float4 s1[BATCH];  // c/2, opacity, max(r,0), max(g,0)
-  __shared__ float2 s2[BATCH];  // max(b,0), exp-skip bound
+  // p0 = (x, y, a/2, b), p1 = (c/2, opacity, max(r,0), max(g,0)),
+  // p2 = (max(b,0), exp-skip bound, blended flag (bits), unused)
+  __shared__ StagedSplat rec[BATCH];
   __shared__ uint32_t sg[BATCH];
-  __shared__ uint32_t sv[BATCH];
   __shared__ uint64_t swords[(RW / 8) * (RW / 8)][BATCH / 64];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
@@ -321,11 +324,10 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
         if (j < cnt) {
           const uint32_t inst = A.isect_vals[b + j];
           const ProjSplat q = A.proj[inst];
-          s0[j] = q.p0;
-          s1[j] = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
-          s2[j] = make_float2(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin);
+          rec[j].p0 = q.p0;
+          rec[j].p1 = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
+          rec[j].p2 = make_float4(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin, 0.0f, 0.0f);
           sg[j] = inst - (uint32_t)v * A.n;
-          sv[j] = 0u;
           bits = A.cull ? block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy)
                         : (1u << ((RW / 8) * (RW / 8))) - 1u;
         }
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
       // Wave-uniform loop: every lane walks the splats that can reach its
       // wave's 8x8 block until the whole wave is done; a finished lane only
       // stops updating its pixel.
-      auto composite = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
+      auto composite = [&](const float4 a0, const float4 a1, const float4 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a0.w * dx) * dy) + (a1.x * dy) * dy;
         if (COUNT) {
@@ -354,18 +356,19 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
           cg = cg + a1.w * vis;
           cb = cb + a2.x * vis;
           T = T * (1.0f - alpha);
-          sv[j] = 1u;
+          // flag in the record's third word; other waves read only its x, y
+          reinterpret_cast<uint32_t*>(&rec[j].p2)[2] = 1u;
           pen = T < RGS_F(1e-4) ? kInf : pen;
         }
       };
-      for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, composite);
+      for_each_reaching<BATCH>(swords[tid >> 6], cnt, rec, [&] { return pen == 0.0f; }, composite);
       __syncthreads();
       if (A.visible)
         for (uint32_t j = tid; j < cnt; j += NT)
-          if (sv[j]) A.visible[sg[j]] = 1.0f;
+          if (__float_as_uint(rec[j].p2.z)) A.visible[sg[j]] = 1.0f;
       if (A.vis_inst)
         for (uint32_t j = tid; j < cnt; j += NT)
-          if (sv[j]) A.vis_inst[(size_t)v * A.n + sg[j]] = 1u;
+          if (__float_as_uint(rec[j].p2.z)) A.vis_inst[(size_t)v * A.n + sg[j]] = 1u;
     }
     float l1 = 0.0f;
     if (inside) {
@@ -406,27 +409,45 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
   // 10-11 |v_xy|, 12 coverage count.
   constexpr int NA = 13;
   constexpr int SA = NA;  // LDS stride per splat (13 floats: less LDS per workgroup, more resident waves)
-  __shared__ float4 s0[BATCH];  // x, y, a/2, b
-  // colours as stored by clamp_colour(): max(c, 0), with the sign bit set
-  // where c < 0 (the colour clamp's gradient gate)
-  __shared__ float4 s1[BATCH];  // c/2, opacity, r, g
-  __shared__ float2 s2[BATCH];  // b, exp-skip bound
-  __shared__ uint32_t si[BATCH];
+  // p0 = (x, y, a/2, b), p1 = (c/2, opacity, r, g), p2 = (b, exp-skip bound,
+  // c, instance id (bits)); colours as stored by clamp_colour(): max(col, 0),
+  // with the sign bit set where col < 0 (the colour clamp's gradient gate)
+  __shared__ StagedSplat rec[BATCH];
   __shared__ float sacc[BATCH * SA];  // all zero between batches
   __shared__ uint32_t sfl[BATCH];     // staged splats with contributions
   __shared__ uint32_t s_wc[NT / 64];
+  // LDS accumulator position p of a splat row -> its slot in the per-instance
+  // accumulator: lane 16g+15 adds its three totals at positions 3g..3g+2
+  // (one address, three immediate offsets); s_pos_cov is the position of
+  // the coverage count
+  __shared__ uint32_t s_perm[SA];
+  __shared__ uint32_t s_pos_cov;
   __shared__ uint64_t swords[(RW / 8) * (RW / 8)][BATCH / 64];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   uint32_t ox, oy;
   pixel_in_tile<RW>(tid, &ox, &oy);
-  int slot[3];
-  wave_reduce12_slots(slot);
+  {
+    int slot[3];
+    wave_reduce12_slots(slot);
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-    if (!REFINE && slot[a] == 9) slot[a] = 12;  // coverage count rides in the refine slot
+    for (int a = 0; a < 3; ++a)
+      if (!REFINE && slot[a] == 9) slot[a] = 12;  // coverage count rides in the refine slot
+    if (tid < 64 && (lane & 15) == 15)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        s_perm[3 * (lane >> 4) + a] = (uint32_t)slot[a];
+        if (slot[a] == 12) s_pos_cov = 3u * (uint32_t)(lane >> 4) + (uint32_t)a;
+      }
+    if (tid == 0) {
+      s_perm[12] = 12u;  // REFINE: the coverage count (from lane 63), else never written
+      if (REFINE) s_pos_cov = 12u;
+    }
+  }
   for (uint32_t k = threadIdx.x; k < (uint32_t)(BATCH * SA); k += NT) sacc[k] = 0.0f;
+  __syncthreads();
+  const uint32_t pos_cov = s_pos_cov;
   uint32_t iter = 0, item;
   while (next_item(A, A.tile_count * (SUB * SUB), &s_item, iter, &item)) {
     const uint32_t tile = A.tile_first + item / (SUB * SUB), sub = item % (SUB * SUB);
@@ -484,10 +505,9 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         if (j < cnt) {
           const uint32_t inst = A.isect_vals[b + j];
           const ProjSplat q = A.proj[inst];
-          s0[j] = q.p0;
-          s1[j] = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
-          s2[j] = make_float2(clamp_colour(q.p2.x), q.p2.y + kSkipMargin);
-          si[j] = inst;
+          rec[j].p0 = q.p0;
+          rec[j].p1 = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));
+          rec[j].p2 = make_float4(clamp_colour(q.p2.x), q.p2.y + kSkipMargin, 2.0f * q.p1.x, __uint_as_float(inst));
           bits = A.cull ? block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy)
                         : (1u << ((RW / 8) * (RW / 8))) - 1u;
         }
@@ -495,7 +515,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
       }
       __syncthreads();
       // Same walk as the forward: the splats that can reach the wave's block.
-      auto accumulate = [&](const float4 a0, const float4 a1, const float2 a2, uint32_t j) {
+      auto accumulate = [&](const float4 a0, const float4 a1, const float4 a2, uint32_t j) {
         const float dx = a0.x - pcx, dy = a0.y - pcy;
         const float sigma = ((a0.z * dx) * dx + (a1.x * dy) * dy) + (a0.w * dx) * dy;
         if (COUNT) {
@@ -506,10 +526,11 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         if (!wave_any(need)) return;
         const float gauss = rgs_exp_core_fast(-sigma);
         const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
-        const bool pass = need && !(sigma < 0.0f || alpha < RGS_F(1.0 / 255.0));
+        // bitwise, not short-circuit: lane masks combine without branches
+        const bool pass = need & !((sigma < 0.0f) | (alpha < RGS_F(1.0 / 255.0)));
         const float nT = T * (1.0f - alpha);
-        const bool stop = pass && nT <= RGS_F(1e-4);
-        const bool contrib = pass && !stop;
+        const bool stop = pass & (nT <= RGS_F(1e-4));
+        const bool contrib = pass & !stop;
         pen = stop ? kInf : pen;
         const uint64_t m = __builtin_amdgcn_ballot_w64(contrib);
         if (m == 0) return;
@@ -539,10 +560,10 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         P = P + vis * Q;  // po includes this splat's own colour
         const float v_alpha_all = T * Q + (P - R) * ra;
         // the alpha clamp gates the geometric terms (rasterize_backwards.wgsl:129-143)
-        const bool geo = contrib && a1.y * gauss <= RGS_F(0.999);
+        const bool geo = contrib & (a1.y * gauss <= RGS_F(0.999));
         const float v_alpha = geo ? v_alpha_all : 0.0f;
         const float v_sigma = -alpha * v_alpha;  // alpha is finite on every lane
-        const float ca = 2.0f * a0.z, cb = a0.w, cc = 2.0f * a1.x;
+        const float ca = 2.0f * a0.z, cb = a0.w, cc = a2.z;
         const float sdx = v_sigma * dx, sdy = v_sigma * dy;
         const float vx = ca * sdx + cb * sdy;
         const float vy = cb * sdx + cc * sdy;
@@ -566,22 +587,22 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         float tot[3];
         wave_reduce12(val, tot);
         if ((lane & 15) == 15) {
-          float* dst = sacc + j * SA;
+          float* dst = sacc + j * SA + 3 * (lane >> 4);
 #pragma unroll
-          for (int a = 0; a < 3; ++a) atomicAdd(dst + slot[a], tot[a]);
-          if (REFINE && lane == 63) atomicAdd(dst + 12, (float)__popcll(m));
+          for (int a = 0; a < 3; ++a) atomicAdd(dst + a, tot[a]);
+          if (REFINE && lane == 63) atomicAdd(sacc + j * SA + 12, (float)__popcll(m));
         }
       };
-      for_each_reaching<BATCH>(swords[tid >> 6], cnt, s0, s1, s2, [&] { return pen == 0.0f; }, accumulate);
+      for_each_reaching<BATCH>(swords[tid >> 6], cnt, rec, [&] { return pen == 0.0f; }, accumulate);
       __syncthreads();
-      // Flush the splats that received contributions (coverage slot 12 is
+      // Flush the splats that received contributions (the coverage count is
       // nonzero exactly then): 13 consecutive threads per splat, one float
       // each, so the atomics of a splat go to one 64-byte accumulator line.
-      // Each slot is cleared as it is read.
+      // Each position is cleared as it is read.
       uint32_t nf = 0;
       for (uint32_t j0 = 0; j0 < (uint32_t)BATCH; j0 += NT) {
         const uint32_t j = j0 + tid;
-        const bool has = j < cnt && sacc[j * SA + 12] != 0.0f;
+        const bool has = j < cnt && sacc[j * SA + pos_cov] != 0.0f;
         uint32_t tot;
         // one call per batch when BATCH <= NT: the next write of s_wc comes
         // after the barriers of the next batch
@@ -591,15 +612,15 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
       }
       __syncthreads();
       for (uint32_t e = tid; e < nf * SA; e += NT) {
-        const uint32_t j = sfl[e / SA], k = e % SA;
-        float x = sacc[j * SA + k];
-        sacc[j * SA + k] = 0.0f;
+        const uint32_t j = sfl[e / SA], pk = e % SA, k = s_perm[pk];
+        float x = sacc[j * SA + pk];
+        sacc[j * SA + pk] = 0.0f;
         if (k == 2 || k == 4) x *= 0.5f;
         if (k >= 5 && k <= 7) {
-          const float col = k == 5 ? s1[j].z : (k == 6 ? s1[j].w : s2[j].x);
+          const float col = k == 5 ? rec[j].p1.z : (k == 6 ? rec[j].p1.w : rec[j].p2.x);
           if (__float_as_int(col) < 0) x = 0.0f;  // clamped colour: no gradient
         }
-        if (x != 0.0f) atomicAdd(A.acc + (size_t)si[j] * kAcc + k, x);
+        if (x != 0.0f) atomicAdd(A.acc + (size_t)__float_as_uint(rec[j].p2.w) * kAcc + k, x);
       }
     }
     if (COUNT) {

@@ -46,13 +46,14 @@ def test_config4_full_step_at_shipped_config(hip_ctx, orc):
     hip_ctx.set_raster(32, 256, True)
     try:
         ds = DeviceSplats(hip_ctx, init)
+        skipped0 = hip_ctx.skipped_steps()  # the context is shared with tests that skip on purpose
         say("config4: 1,048,576 Gaussians, 8 views, tile 32 batch 256 persistent, graph")
         imgs = [im.cpu().numpy() for im in render_forward(hip_ctx, ds, cams)]
         w = 1.0 / len(cams)
         tr = Trainer(hip_ctx, training_optimizer(3, **LRS), loss_weight=w)
         losses = tr.train_step(ds, cams, tgt, batched=True, graph=True, read_loss=True)
         vt, vs, vo, _ = ds.grads()
-        assert hip_ctx.skipped_steps() == 0
+        assert hip_ctx.skipped_steps() == skipped0
         rvt = np.zeros_like(vt)
         rvs = np.zeros_like(vs)
         rvo = np.zeros_like(vo)
