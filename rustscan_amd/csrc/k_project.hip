@@ -195,9 +195,6 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
   const float q0 = P[3 * npad + gl], q1 = P[4 * npad + gl], q2 = P[5 * npad + gl], q3 = P[6 * npad + gl];
   const float sc[3] = {rgs_exp(P[7 * npad + gl]), rgs_exp(P[8 * npad + gl]), rgs_exp(P[9 * npad + gl])};
   const float sig = rgs_sigmoid(P[10 * npad + gl]);
-  float sh[NK * 3];
-#pragma unroll
-  for (int i = 0; i < NK * 3; ++i) sh[i] = P[(size_t)(11 + i) * npad + gl];
   const float qn = ((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3;
   const bool qok = live && !(qn < RGS_F(1e-6));
   const float is = 1.0f / rgs_sqrt(qn);
@@ -252,6 +249,15 @@ __global__ __launch_bounds__(256) void k_project_fwd(const float* __restrict__ P
           const float dx = mean[0] - V.campos[0], dy = mean[1] - V.campos[1], dz = mean[2] - V.campos[2];
           const float dd = (dx * dx + dy * dy) + dz * dz;
           const float rs = 1.0f / rgs_sqrt(fmaxf_(dd, RGS_F(1e-12)));
+          // The SH coefficients are read per passing view rather than held
+          // across the view loop: 48 live registers at degree 3 would halve
+          // the kernel's occupancy.  The empty asm hides the address from
+          // the optimiser so it cannot hoist the loads out of the loop again.
+          const float* shp = P + (size_t)11 * npad + gl;
+          asm volatile("" : "+v"(shp));
+          float sh[NK * 3];
+#pragma unroll
+          for (int i = 0; i < NK * 3; ++i) sh[i] = shp[(size_t)i * npad];
           float col[3];
           sh_color<DEG>(sh, dx * rs, dy * rs, dz * rs, col);
           ProjSplat& ps = proj[inst];
