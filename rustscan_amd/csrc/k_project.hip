@@ -466,6 +466,26 @@ struct S3 {
   }
 };
 
+// The Gaussian's frame in double: raw quaternion qu, |qu|^2, |qu|, the unit
+// quaternion q, its rotation (columns rot[c]) and the squared scales
+// (project_backwards.wgsl:57-63).
+__device__ __forceinline__ void gauss_frame64(const float* P, uint32_t npad, uint32_t gid, double qu[4], double* qn,
+                                              double* qlen, double q[4], double rot[3][3], double s2[3]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) qu[i] = (double)P[(3 + i) * npad + gid];
+  *qn = qu[0] * qu[0] + qu[1] * qu[1] + qu[2] * qu[2] + qu[3] * qu[3];
+  *qlen = sqrt(fmax(*qn, 1e-20));
+  const double qis = 1.0 / *qlen;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = qu[i] * qis;
+  quat_rot64(q, rot);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double sc = exp((double)P[(7 + c) * npad + gid]);
+    s2[c] = sc * sc;
+  }
+}
+
 // grads: SoA planes like the parameters (59 planes at SH3).
 // stats: [7][npad] screen-gradient statistics (project_backwards.wgsl:33-36,106-108), summed over views.
 // accumulate != 0 adds into grads/stats/dinc instead of overwriting (per-view launches).
@@ -481,8 +501,11 @@ struct S3 {
 // Symmetric matrices of the chain (covariances and their gradients) are
 // handled as symmetric: the reference forms them as literal products, whose
 // asymmetric rounding is below FP64's.
+// Three waves per SIMD (168 VGPRs, a 12-byte spill): the kernel streams the
+// accumulator lines and parameter planes, and the third wave hides more of
+// that latency than the spill costs (0.40 -> 0.32 ms on 8 views x 1M).
 template <int DEG>
-__global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P, uint32_t npad, uint32_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_project_bwd(const float* __restrict__ P, uint32_t npad, uint32_t n,
                                                      const ViewDev* __restrict__ views, int v0, int nv, float blur,
                                                      float* __restrict__ acc, const uint32_t* __restrict__ keys_by_inst,
                                                      float* __restrict__ grads, float* __restrict__ stats,
@@ -548,30 +571,24 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
   }
 
   // Pass 2: opacity, screen statistics and the covariance chain in FP64;
-  // consumes and clears the accumulator lines.
+  // consumes and clears the accumulator lines.  The covariance gradient is
+  // linear in each view's v_cov2d, so it is summed over the views in world
+  // space (v_cov3d = Rv^T vcc Rv) and the scale and rotation VJPs are taken
+  // once, after the loop; the loop itself needs only the world covariance.
   const float opac = rgs_sigmoid(P[10 * npad + gid]);
-  const double qu[4] = {(double)P[3 * npad + gid], (double)P[4 * npad + gid], (double)P[5 * npad + gid],
-                        (double)P[6 * npad + gid]};
-  // quaternion normalisation and scales (project_backwards.wgsl:57-63)
-  const double qn = qu[0] * qu[0] + qu[1] * qu[1] + qu[2] * qu[2] + qu[3] * qu[3];
-  const double qlen = sqrt(fmax(qn, 1e-20));
-  double q[4];
+  S3 cw;  // world covariance rot diag(s^2) rot^T
   {
-    const double qis = 1.0 / qlen;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) q[i] = qu[i] * qis;
+    double qu[4], qn, qlen, q[4], rot[3][3], s2[3];
+    gauss_frame64(P, npad, gid, qu, &qn, &qlen, q, rot, s2);
+    auto cov = [&](int i, int j) { return rot[0][i] * rot[0][j] * s2[0] + rot[1][i] * rot[1][j] * s2[1] + rot[2][i] * rot[2][j] * s2[2]; };
+    cw.xx = cov(0, 0); cw.yy = cov(1, 1); cw.zz = cov(2, 2);
+    cw.xy = cov(0, 1); cw.xz = cov(0, 2); cw.yz = cov(1, 2);
   }
-  double rot[3][3];
-  quat_rot64(q, rot);
-  double s2[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double sc = exp((double)P[(7 + c) * npad + gid]);
-    s2[c] = sc * sc;
-  }
-  float gmean[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f};
+  float gmean[3] = {0.f, 0.f, 0.f};
   float gop = 0.f;
   float st[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  S3 w3 = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // v_cov3d summed over views
+  bool chain = false;
   for (int v = v0; v < v0 + nv; ++v) {
     const size_t inst = (size_t)v * n + gid;
     if (keys_by_inst[inst] == 0xffffffffu) continue;
@@ -595,32 +612,35 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
       st[5] += a3.x;
       st[6] += a3.x * mcz_f;
     }
-    // B = Rv * rot (the Gaussian's axes in camera space); cc = B diag(s^2) B^T
-    double B[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-        B[c][r] = (double)V.vm[r] * rot[c][0] + (double)V.vm[4 + r] * rot[c][1] + (double)V.vm[8 + r] * rot[c][2];
+    chain = true;
+    // view rotation Rv[i][k] = vm[4k + i]
+    auto Rv = [&](int i, int k) { return (double)V.vm[4 * k + i]; };
+    // camera-space covariance cc = Rv cw Rv^T
     S3 cc;
-    cc.xx = B[0][0] * B[0][0] * s2[0] + B[1][0] * B[1][0] * s2[1] + B[2][0] * B[2][0] * s2[2];
-    cc.yy = B[0][1] * B[0][1] * s2[0] + B[1][1] * B[1][1] * s2[1] + B[2][1] * B[2][1] * s2[2];
-    cc.zz = B[0][2] * B[0][2] * s2[0] + B[1][2] * B[1][2] * s2[1] + B[2][2] * B[2][2] * s2[2];
-    cc.xy = B[0][0] * B[0][1] * s2[0] + B[1][0] * B[1][1] * s2[1] + B[2][0] * B[2][1] * s2[2];
-    cc.xz = B[0][0] * B[0][2] * s2[0] + B[1][0] * B[1][2] * s2[1] + B[2][0] * B[2][2] * s2[2];
-    cc.yz = B[0][1] * B[0][2] * s2[0] + B[1][1] * B[1][2] * s2[1] + B[2][1] * B[2][2] * s2[2];
+    {
+      double t[3][3];  // t[i][l] = (Rv cw)[i][l]
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) t[i][l] = Rv(i, 0) * cw.at(0, l) + Rv(i, 1) * cw.at(1, l) + Rv(i, 2) * cw.at(2, l);
+      auto ct = [&](int i, int j) { return t[i][0] * Rv(j, 0) + t[i][1] * Rv(j, 1) + t[i][2] * Rv(j, 2); };
+      cc.xx = ct(0, 0); cc.yy = ct(1, 1); cc.zz = ct(2, 2);
+      cc.xy = ct(0, 1); cc.xz = ct(0, 2); cc.yz = ct(1, 2);
+    }
     // camera-space mean
     double mc[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
       mc[r] = (double)V.vm[r] * mean[0] + (double)V.vm[4 + r] * mean[1] + (double)V.vm[8 + r] * mean[2] +
               (double)V.vm[12 + r];
-    // J = [[jx, 0, j20], [0, jy, j21]] with the EWA clip (helpers.wgsl:138-155)
+    // J = [[jx, 0, j20], [0, jy, j21]] with the EWA clip (helpers.wgsl:138-155).
+    // mc[2] >= 0.01 up to the rounding of the float test above, so the
+    // reference's 1/max(mc_z, 0.01) and 1/max(|mc_z|, 0.01) are one value.
     const double fx = V.fx, fy = V.fy;
     const double lpx = V.lim_px, lpy = V.lim_py, lnx = V.lim_nx, lny = V.lim_ny;
-    const double rzj = 1.0 / fmax(mc[2], 0.01);
-    const double jx = fx * rzj, jy = fy * rzj;
-    const double j20 = -jx * fmin(fmax(mc[0] * rzj, lnx), lpx), j21 = -jy * fmin(fmax(mc[1] * rzj, lny), lpy);
+    const double rz = 1.0 / fmax(fabs(mc[2]), 0.01);
+    const double jx = fx * rz, jy = fy * rz;
+    const double j20 = -jx * fmin(fmax(mc[0] * rz, lnx), lpx), j21 = -jy * fmin(fmax(mc[1] * rz, lny), lpy);
     // cov2d = J cc J^T
     const double c2xx = jx * (jx * cc.xx + j20 * cc.xz) + j20 * (jx * cc.xz + j20 * cc.zz);
     const double c2yy = jy * (jy * cc.yy + j21 * cc.yz) + j21 * (jy * cc.yz + j21 * cc.zz);
@@ -651,7 +671,6 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     }
     // persp_proj_vjp (helpers.wgsl:382-427); v_J = 2 vc J cc for symmetric vc and cc
     const double x = mc[0], y = mc[1];
-    const double rz = 1.0 / fmax(fabs(mc[2]), 0.01);
     const double rz2 = rz * rz;
     double vm0 = fx * rz * (double)a0.x;
     double vm1 = fy * rz * (double)a0.y;
@@ -695,28 +714,38 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     vcc.xz = jx * p20;
     vcc.yz = jy * p21;
     vcc.zz = j20 * p20 + j21 * p21;
-    // In the Gaussian's frame, G = B^T vcc B (G[a][b] = B[a] . vcc B[b]).  With
-    // v_cov3d = Rv^T vcc Rv and m = rot diag(s):
-    //   v_m[c] = 2 v_cov3d rot[c] s_c,  v_scale[c] = rot[c] . v_m[c] = 2 G[c][c] s_c,
-    //   v_rot[c] = v_m[c] s_c = 2 s_c^2 sum_b G[b][c] rot[b]
-    double G[3][3];
+    // v_cov3d = Rv^T vcc Rv, added to the world-space sum
+    {
+      double u[3][3];  // u[i][l] = (vcc Rv)[i][l]
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      double vb[3];
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) vb[r] = vcc.at(r, 0) * B[b][0] + vcc.at(r, 1) * B[b][1] + vcc.at(r, 2) * B[b][2];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) G[a][b] = B[a][0] * vb[0] + B[a][1] * vb[1] + B[a][2] * vb[2];
+        for (int l = 0; l < 3; ++l) u[i][l] = vcc.at(i, 0) * Rv(0, l) + vcc.at(i, 1) * Rv(1, l) + vcc.at(i, 2) * Rv(2, l);
+      auto wt = [&](int k, int l) { return Rv(0, k) * u[0][l] + Rv(1, k) * u[1][l] + Rv(2, k) * u[2][l]; };
+      w3.xx += wt(0, 0); w3.yy += wt(1, 1); w3.zz += wt(2, 2);
+      w3.xy += wt(0, 1); w3.xz += wt(0, 2); w3.yz += wt(1, 2);
     }
-    // log-scale: dL/dlog s = s dL/ds
-#pragma unroll
-    for (int c = 0; c < 3; ++c) gls[c] += (float)(2.0 * G[c][c] * s2[c]);
+  }
+
+  // scale and rotation VJPs of the summed v_cov3d.  With m = rot diag(s):
+  //   v_m[c] = 2 v_cov3d rot[c] s_c,  v_rot[c] = v_m[c] s_c = 2 s_c^2 v_cov3d rot[c],
+  //   dL/dlog s_c = s_c rot[c] . v_m[c] = rot[c] . v_rot[c].
+  // The frame is read again through an opaque pointer so that none of it is
+  // held in registers across the view loop.
+  float gq[4] = {0.f, 0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f};
+  if (chain) {
+    const float* Pf = P;
+    asm volatile("" : "+s"(Pf));
+    double qu[4], qn, qlen, q[4], rot[3][3], s2[3];
+    gauss_frame64(Pf, npad, gid, qu, &qn, &qlen, q, rot, s2);
     double vr[3][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int r = 0; r < 3; ++r)
-        vr[c][r] = 2.0 * s2[c] * (G[0][c] * rot[0][r] + G[1][c] * rot[1][r] + G[2][c] * rot[2][r]);
+        vr[c][r] = 2.0 * s2[c] * (w3.at(r, 0) * rot[c][0] + w3.at(r, 1) * rot[c][1] + w3.at(r, 2) * rot[c][2]);
+      gls[c] = (float)(rot[c][0] * vr[c][0] + rot[c][1] * vr[c][1] + rot[c][2] * vr[c][2]);
+    }
     // quat_to_mat3_vjp (helpers.wgsl:82-103)
     const double w = q[0], qx = q[1], qy = q[2], qz = q[3];
     double dq[4];
@@ -728,12 +757,10 @@ __global__ __launch_bounds__(256) void k_project_bwd(const float* __restrict__ P
     dq[3] = 2.0 * (qx * (vr[0][2] + vr[2][0]) + qy * (vr[1][2] + vr[2][1]) - 2.0 * qz * (vr[0][0] + vr[1][1]) +
                    w * (vr[0][1] - vr[1][0]));
     // normalize_vjp(qu) (helpers.wgsl:105-119): (|q|^2 I - q q^T) / |q|^3
-    {
-      const double sN = 1.0 / (qlen * qn);
-      const double qd = qu[0] * dq[0] + qu[1] * dq[1] + qu[2] * dq[2] + qu[3] * dq[3];
+    const double sN = 1.0 / (qlen * qn);
+    const double qd = qu[0] * dq[0] + qu[1] * dq[1] + qu[2] * dq[2] + qu[3] * dq[3];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) gq[r] += (float)((qn * dq[r] - qu[r] * qd) * sN);
-    }
+    for (int r = 0; r < 4; ++r) gq[r] = (float)((qn * dq[r] - qu[r] * qd) * sN);
   }
 
   out(0, gmean[0]); out(1, gmean[1]); out(2, gmean[2]);
