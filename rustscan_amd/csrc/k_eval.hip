@@ -21,13 +21,13 @@
 //   k_eval_expand        one thread per (Gaussian, tile) instance
 //   (radix sort of tile ids, stable)  per-tile lists in depth order
 //   k_eval_ranges        [start, end) of every tile's list
-//   k_eval_raster        one 16x16 workgroup per tile
+//   k_eval_raster        one 16x16 workgroup per tile, an 8x8 block per wave
 // Metrics (evaluation/core.rs:238-319): k_eval_metrics sums the float32
 // per-pixel terms of PSNR and the two sharpness measures in float64 per
 // workgroup, k_eval_metrics_final adds the workgroup sums in a fixed order.
 #include <cfloat>
 
-#include "rgs_internal.h"
+#include "rgs_device.h"
 #include "rgs_math.h"
 
 namespace rgs {
@@ -267,19 +267,28 @@ __global__ __launch_bounds__(256) void k_eval_ranges(const uint32_t* __restrict_
   if (i + 1 == T || keys[i + 1] != t) ranges[2 * (size_t)t + 1] = i + 1;
 }
 
-// One workgroup per 16x16 tile, one pixel per thread; the tile's list is
-// staged through LDS 256 records at a time.  A pixel stops once its alpha
-// reaches 0.999 (the reference skips it for every later splat, and alpha
-// never decreases); the workgroup stops when every pixel has.
+// One workgroup per 16x16 tile, one pixel per thread; each 64-lane wave owns
+// an 8x8 block of the tile.  The tile's list is staged through LDS 256
+// records at a time, with a mask of the blocks each record's 3-sigma ellipse
+// can reach (block_touch_mask on the Mahalanobis form, bound d^2/2 < 4.5:
+// every pixel centre of an unmarked block provably has d^2 >= 9, where the
+// reference skips the splat).  A wave walks only the records that reach its
+// block, in list order, with the reference's per-pixel arithmetic.  A pixel
+// stops once its alpha reaches 0.999 (the reference skips it for every later
+// splat, and alpha never decreases); a wave stops when all its pixels have,
+// the workgroup when every pixel has.
 __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ rec_s, const uint32_t* __restrict__ vals,
                                                      const uint32_t* __restrict__ ranges, uint32_t W, uint32_t H,
                                                      uint32_t ntx, float bg0, float bg1, float bg2,
                                                      float* __restrict__ color, uint8_t* __restrict__ color8,
                                                      float* __restrict__ depth, int depth_zero) {
   __shared__ float4 sa[256], sb[256], sc[256];
+  __shared__ uint64_t swords[4][4];  // [block][record / 64]: bit record % 64 reaches the block
   const uint32_t tile = blockIdx.x;
   const uint32_t tx = tile % ntx, ty = tile / ntx;
-  const uint32_t px = tx * kEvalTile + (threadIdx.x & 15u), py = ty * kEvalTile + (threadIdx.x >> 4);
+  const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t tox = tx * kEvalTile, toy = ty * kEvalTile;
+  const uint32_t px = tox + (wv & 1u) * 8u + (lane & 7u), py = toy + (wv >> 1) * 8u + (lane >> 3);
   const bool inside = px < W && py < H;
   const float fpx = (float)px + 0.5f, fpy = (float)py + 0.5f;
   float alpha = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, dep = FLT_MAX;
@@ -288,20 +297,36 @@ __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ 
   for (uint32_t base = start; base < end; base += 256) {
     if (__syncthreads_count(done ? 0 : 1) == 0) break;
     const uint32_t i = base + threadIdx.x;
+    uint32_t bits = 0;
     if (i < end) {
       const uint32_t r = vals[i];
-      sa[threadIdx.x] = rec_s[3 * (size_t)r];
-      sb[threadIdx.x] = rec_s[3 * (size_t)r + 1];
+      const float4 A = rec_s[3 * (size_t)r], B = rec_s[3 * (size_t)r + 1];
+      sa[threadIdx.x] = A;
+      sb[threadIdx.x] = B;
       sc[threadIdx.x] = rec_s[3 * (size_t)r + 2];
+      // d^2 = a dx^2 + 2 b dx dy + c dy^2 with the conic of {cxx, 2 cxy, cyy, 1/det}
+      const float a = B.z * B.w, b = -0.5f * B.y * B.w, c = B.x * B.w;
+      bits = block_touch_mask<kEvalTile>(make_float4(A.x, A.y, 0.5f * a, b), 0.5f * c, 4.5f, tox, toy);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t m = __builtin_amdgcn_ballot_w64((bits >> k) & 1u);
+      if (lane == 0) swords[k][wv] = m;
     }
     __syncthreads();
-    const uint32_t m = min(256u, end - base);
-    if (!done) {
-      for (uint32_t j = 0; j < m; ++j) {
+    const uint32_t nw = (min(256u, end - base) + 63u) / 64u;
+    for (uint32_t wi = 0; wi < nw; ++wi) {
+      const uint64_t wm = swords[wv][wi];
+      uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(wm >> 32)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)wm);
+      while (m != 0) {
+        if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+        const uint32_t j = wi * 64u + (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
         const float4 A = sa[j], B = sb[j];
         const float dx = A.x - fpx, dy = A.y - fpy;
         const float d2 = (B.z * dx * dx - B.y * dx * dy + B.x * dy * dy) * B.w;
-        if (d2 < 9.0f) {
+        if (!done && d2 < 9.0f) {
           const float w = rgs_exp(-0.5f * d2) * A.z;
           if (w > 0.001f) {
             const float a = w * (1.0f - alpha);
@@ -312,15 +337,13 @@ __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ 
               c2 = c2 + Cc.z * a;
               alpha = alpha + a;
               if (dep == FLT_MAX && a > 0.01f) dep = A.w;
-              if (alpha >= 0.999f) {
-                done = true;
-                break;
-              }
+              if (alpha >= 0.999f) done = true;
             }
           }
         }
       }
     }
+    __syncthreads();
   }
   if (!inside) return;
   const float t = 1.0f - alpha;

@@ -200,4 +200,60 @@ __device__ __forceinline__ float wave_sum(float x) {
   return x;
 }
 
+// Which 8x8 pixel blocks of a workgroup a staged splat can reach.  Bit
+// bx + by*(RW/8) is cleared only when every pixel centre of block (bx, by)
+// provably has sigma > bound, so the wave of that block would find no pixel
+// to composite and may skip the splat without changing any result.
+// The bound is the exact minimum of q(dx, dy) = (a dx^2 + 2b dx dy + c dy^2)/2
+// over the block's rectangle of (dx, dy) (rounding is monotone, so the float
+// dx, dy of every pixel centre lie between those of the extreme centres).
+// For a positive-definite conic (a, c > 0, ac - b^2 > 0) q is convex and
+// 2-homogeneous, so when the rectangle misses the origin its minimum lies on
+// an edge facing the origin (scaling any point towards the origin lowers q
+// and meets such an edge inside the rectangle); along the edge dx = X the
+// minimum is at dy = clamp(-bX/c, Y0, Y1).  The argmin is found with the
+// hardware reciprocal: its relative error moves q by second-order amounts.
+// The float sigma of a pixel and the float edge minimum each lie within
+// 4.8e-7*(a dx^2 + c dy^2) of their exact values (four roundings per term,
+// 2|b dx dy| <= a dx^2 + c dy^2); E = 2e-6*(a xm^2 + c ym^2), with xm, ym the
+// largest |dx|, |dy| over the workgroup, covers both.  NaN or
+// non-positive-definite conics keep every bit.
+template <int RW>
+__device__ __forceinline__ uint32_t block_touch_mask(const float4 q0, float hc, float bound, uint32_t tox,
+                                                     uint32_t toy) {
+  constexpr int BPR = RW / 8;
+  constexpr uint32_t kAll = (1u << (BPR * BPR)) - 1u;
+  const float a = 2.0f * q0.z, b = q0.w, c = 2.0f * hc;
+  const float det_lo = (a * c) * (1.0f - 1e-6f) - (b * b) * (1.0f + 1e-6f);
+  if (!(a > 0.0f && c > 0.0f && det_lo > 0.0f)) return kAll;
+  const float ra = __builtin_amdgcn_rcpf(a), rc = __builtin_amdgcn_rcpf(c);
+  // dx = x - pcx over the pixel centres pcx = tox + 8i + {0.5 .. 7.5}
+  float x0[BPR], x1[BPR], y0[BPR], y1[BPR];
+#pragma unroll
+  for (int i = 0; i < BPR; ++i) {
+    x0[i] = q0.x - ((float)(tox + 8 * i + 7) + 0.5f);
+    x1[i] = q0.x - ((float)(tox + 8 * i) + 0.5f);
+    y0[i] = q0.y - ((float)(toy + 8 * i + 7) + 0.5f);
+    y1[i] = q0.y - ((float)(toy + 8 * i) + 0.5f);
+  }
+  const float xm = fmaxf_(fabsf(x0[0]), fabsf(x1[BPR - 1])), ym = fmaxf_(fabsf(y0[0]), fabsf(y1[BPR - 1]));
+  const float lim = bound + 2e-6f * (a * xm * xm + c * ym * ym);
+  uint32_t bits = 0;
+#pragma unroll
+  for (int by = 0; by < BPR; ++by)
+#pragma unroll
+    for (int bx = 0; bx < BPR; ++bx) {
+      const float X0 = x0[bx], X1 = x1[bx], Y0 = y0[by], Y1 = y1[by];
+      const bool fx = X0 > 0.0f || X1 < 0.0f, fy = Y0 > 0.0f || Y1 < 0.0f;
+      const float Xe = X0 > 0.0f ? X0 : X1, Ye = Y0 > 0.0f ? Y0 : Y1;
+      const float dy = __builtin_amdgcn_fmed3f(-b * Xe * rc, Y0, Y1);
+      const float dx = __builtin_amdgcn_fmed3f(-b * Ye * ra, X0, X1);
+      const float qx = 0.5f * ((a * Xe) * Xe + (2.0f * b * Xe + c * dy) * dy);
+      const float qy = 0.5f * ((c * Ye) * Ye + (2.0f * b * Ye + a * dx) * dx);
+      const float m = fx ? (fy ? fminf_(qx, qy) : qx) : (fy ? qy : 0.0f);
+      if (!(m > lim)) bits |= 1u << (bx + by * BPR);
+    }
+  return bits;
+}
+
 }  // namespace rgs
