@@ -543,9 +543,10 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
 // boundaries that fall inside them.
 constexpr uint32_t kRangeKeys = 16;
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ counters, uint32_t cap,
-                                                     uint32_t num_tiles, uint32_t* __restrict__ ranges) {
-  const uint32_t m = min(counters[kCntIsect], cap);
+                                                     const uint32_t* __restrict__ n_dev, uint32_t n_host,
+                                                     uint32_t cap, uint32_t num_tiles,
+                                                     uint32_t* __restrict__ ranges) {
+  const uint32_t m = dev_count(n_dev, n_host, cap);
   const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kRangeKeys;
   if (i0 >= m) return;
   uint32_t k[kRangeKeys];
@@ -721,7 +722,15 @@ void launch_tile_ranges(hipStream_t s, const uint32_t* keys, const uint32_t* cou
                         uint32_t num_tiles, uint32_t* ranges) {
   if (cap == 0) return;
   const uint64_t threads = ((uint64_t)cap + kRangeKeys - 1) / kRangeKeys;
-  k_tile_ranges<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(keys, counters, cap, num_tiles, ranges);
+  k_tile_ranges<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(keys, counters + kCntIsect, 0, cap, num_tiles,
+                                                                   ranges);
+}
+
+// Same over a list of n keys (a host count).
+void launch_tile_ranges_n(hipStream_t s, const uint32_t* keys, uint32_t n, uint32_t num_tiles, uint32_t* ranges) {
+  if (n == 0) return;
+  const uint64_t threads = ((uint64_t)n + kRangeKeys - 1) / kRangeKeys;
+  k_tile_ranges<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(keys, nullptr, n, n, num_tiles, ranges);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {

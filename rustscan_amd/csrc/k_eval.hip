@@ -18,9 +18,9 @@
 //   (scan + k_eval_compact + radix sort of depth bits)  depth order of the
 //                        live Gaussians, ties by index (stable)
 //   k_eval_gather        records and tile boxes into depth order
-//   k_eval_expand        one thread per (Gaussian, tile) instance
+//   k_eval_block_ranks, k_eval_expand  one thread per (Gaussian, tile) instance
 //   (radix sort of tile ids, stable)  per-tile lists in depth order
-//   k_eval_ranges        [start, end) of every tile's list
+//   (tile ranges)        [start, end) of every tile's list
 //   k_eval_raster        one 16x16 workgroup per tile, an 8x8 block per wave
 // Metrics (evaluation/core.rs:238-319): k_eval_metrics sums the float32
 // per-pixel terms of PSNR and the two sharpness measures in float64 per
@@ -235,36 +235,45 @@ __global__ __launch_bounds__(256) void k_eval_gather(const uint32_t* __restrict_
 
 // Instance i belongs to the depth rank r with offs[r] <= i < offs[r + 1]
 // (every live Gaussian has at least one tile, so offs is strictly
-// increasing); its k-th tile in row-major box order.
+// increasing); its k-th tile in row-major box order.  Expansion works on
+// 256-instance blocks; k_eval_block_ranks writes the rank of each block's
+// first instance (rank r writes it for the blocks that start in its range).
+__global__ __launch_bounds__(256) void k_eval_block_ranks(const uint32_t* __restrict__ offs, uint32_t V, uint32_t T,
+                                                          uint32_t* __restrict__ brank) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V) return;
+  const uint32_t s = offs[r], e = r + 1 < V ? offs[r + 1] : T;
+  for (uint32_t b = (s + 255u) / 256u; b * 256u < e; ++b) brank[b] = r;
+}
+
+// The 256 instances of a block belong to at most 256 consecutive ranks
+// (rank r0 + 256 starts past the block): the workgroup stages their offsets
+// in LDS and each thread finds its own rank there.
 __global__ __launch_bounds__(256) void k_eval_expand(const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ brank,
                                                      const uint2* __restrict__ box_s, uint32_t V, uint32_t T,
                                                      uint32_t ntx, uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
+  __shared__ uint32_t s_off[256];
+  const uint32_t r0 = brank[blockIdx.x];
+  s_off[threadIdx.x] = r0 + threadIdx.x < V ? offs[r0 + threadIdx.x] : 0xffffffffu;
+  __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= T) return;
-  uint32_t lo = 0, hi = V - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (offs[mid] <= i)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const uint32_t k = i - offs[lo];
+  // largest q with s_off[q] <= i (s_off[0] <= i)
+  uint32_t q = 0;
+#pragma unroll
+  for (uint32_t step = 128; step > 0; step >>= 1)
+    if (s_off[q + step] <= i) q += step;
+  const uint32_t lo = r0 + q;
+  const uint32_t k = i - s_off[q];
   const uint2 b = box_s[lo];
   const uint32_t txmin = b.x & 0xffffu, txmax = b.x >> 16, tymin = b.y & 0xffffu;
-  const uint32_t wt = txmax - txmin;
-  keys[i] = (tymin + k / wt) * ntx + txmin + k % wt;
+  const uint32_t wt = txmax - txmin;  // >= 1: the box of a live Gaussian covers a tile
+  uint32_t row, col;
+  bbox_rc(k, wt, __builtin_amdgcn_rcpf((float)wt), &row, &col);  // k < 2^20: exact
+  keys[i] = (tymin + row) * ntx + txmin + col;
   vals[i] = lo;
-}
-
-__global__ __launch_bounds__(256) void k_eval_ranges(const uint32_t* __restrict__ keys, uint32_t T,
-                                                     uint32_t* __restrict__ ranges) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= T) return;
-  const uint32_t t = keys[i];
-  if (i == 0 || keys[i - 1] != t) ranges[2 * (size_t)t] = i;
-  if (i + 1 == T || keys[i + 1] != t) ranges[2 * (size_t)t + 1] = i + 1;
 }
 
 // One workgroup per 16x16 tile, one pixel per thread; each 64-lane wave owns
@@ -476,14 +485,14 @@ void launch_eval_gather(hipStream_t s, const uint32_t* order, uint32_t V, const 
   if (V) k_eval_gather<<<(V + 255) / 256, 256, 0, s>>>(order, V, rec, box, rec_s, box_s);
 }
 
+// brank: (T + 255) / 256 words.
 void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t ntx,
-                        uint32_t* keys, uint32_t* vals) {
-  if (T) k_eval_expand<<<(T + 255) / 256, 256, 0, s>>>(offs, box_s, V, T, ntx, keys, vals);
+                        uint32_t* brank, uint32_t* keys, uint32_t* vals) {
+  if (T == 0 || V == 0) return;
+  k_eval_block_ranks<<<(V + 255) / 256, 256, 0, s>>>(offs, V, T, brank);
+  k_eval_expand<<<(T + 255) / 256, 256, 0, s>>>(offs, brank, box_s, V, T, ntx, keys, vals);
 }
 
-void launch_eval_ranges(hipStream_t s, const uint32_t* keys, uint32_t T, uint32_t* ranges) {
-  if (T) k_eval_ranges<<<(T + 255) / 256, 256, 0, s>>>(keys, T, ranges);
-}
 
 void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint32_t* vals, const uint32_t* ranges,
                         const EvalCam& C, const float bg[3], float* color, uint8_t* color8, float* depth,

@@ -22,8 +22,8 @@ void launch_eval_compact(hipStream_t s, const uint32_t* live, const uint32_t* po
 void launch_eval_gather(hipStream_t s, const uint32_t* order, uint32_t V, const float4* rec, const uint2* box,
                         float4* rec_s, uint2* box_s);
 void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t ntx,
-                        uint32_t* keys, uint32_t* vals);
-void launch_eval_ranges(hipStream_t s, const uint32_t* keys, uint32_t T, uint32_t* ranges);
+                        uint32_t* brank, uint32_t* keys, uint32_t* vals);
+void launch_tile_ranges_n(hipStream_t s, const uint32_t* keys, uint32_t n, uint32_t num_tiles, uint32_t* ranges);
 void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint32_t* vals, const uint32_t* ranges,
                         const EvalCam& C, const float bg[3], float* color, uint8_t* color8, float* depth,
                         int depth_zero);
@@ -33,6 +33,9 @@ void launch_eval_metrics(hipStream_t s, const float* rend, const uint8_t* rend8,
 int radix_sort_pairs(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
                      uint32_t* kc, uint32_t* vc, const uint32_t* n_dev, uint32_t n_host, uint32_t cap, int key_bits,
                      uint32_t* scratch, const uint32_t* gsrc, uint32_t* gout, int count_ib);
+int radix_sort_pairs_packed(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kb, uint32_t* vb,
+                            uint32_t* kout, uint32_t* vout, const uint32_t* n_dev, uint32_t n_host, uint32_t cap,
+                            int key_bits, int value_bits, uint32_t* scratch);
 uint64_t radix_scratch_words(uint64_t cap);
 void scan_u32(hipStream_t s, const uint32_t* in, const uint32_t* gather, const uint32_t* n_dev, uint32_t n_host,
               uint32_t cap, uint32_t* out, int inclusive, uint32_t* total_dev, uint32_t* max_dev, uint32_t* scratch);
@@ -134,6 +137,7 @@ int eval_render(rgs_ctx* c, const rgs_splats* s, const rgs_camera* cam, const fl
     rc |= ensure(c, E.ik1, (size_t)Ti * 4);
     rc |= ensure(c, E.iv1, (size_t)Ti * 4);
     rc |= ensure(c, E.sort, radix_scratch_words(std::max<uint64_t>(V, T)) * 4);
+    rc |= ensure(c, E.brank, ((size_t)Ti + 255) / 256 * 4);
     if (rc) return RGS_ERR_HIP;
     uint32_t* sort = E.sort.as<uint32_t>();
     // depth order of the live Gaussians: positive float depths sort as their bits
@@ -145,15 +149,17 @@ int eval_render(rgs_ctx* c, const rgs_splats* s, const rgs_camera* cam, const fl
              E.scan.as<uint32_t>());
     launch_eval_gather(st, order, V, E.rec.as<float4>(), E.box.as<uint2>(), E.rec_s.as<float4>(),
                        E.box_s.as<uint2>());
-    launch_eval_expand(st, E.offs.as<uint32_t>(), E.box_s.as<uint2>(), V, Ti, C.ntx, E.ik0.as<uint32_t>(),
-                       E.iv0.as<uint32_t>());
-    // per-tile lists: stable sort by tile id keeps the depth order inside a tile
-    const int w2 = radix_sort_pairs(st, E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), E.ik1.as<uint32_t>(),
-                                    E.iv1.as<uint32_t>(), E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), nullptr, Ti, Ti,
-                                    (int)ceil_log2(ntiles), sort, nullptr, nullptr, 0);
+    launch_eval_expand(st, E.offs.as<uint32_t>(), E.box_s.as<uint2>(), V, Ti, C.ntx, E.brank.as<uint32_t>(),
+                       E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>());
+    // per-tile lists: stable sort by tile id keeps the depth order inside a
+    // tile (two packed passes when the tile id and the record index fit one
+    // word, as in the training step's tile sort)
+    const int w2 = radix_sort_pairs_packed(st, E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), E.ik1.as<uint32_t>(),
+                                           E.iv1.as<uint32_t>(), E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), nullptr,
+                                           Ti, Ti, (int)ceil_log2(ntiles), std::max(1, (int)ceil_log2(V)), sort);
     const uint32_t* keys = w2 == 1 ? E.ik1.as<uint32_t>() : E.ik0.as<uint32_t>();
     vals = w2 == 1 ? E.iv1.as<uint32_t>() : E.iv0.as<uint32_t>();
-    launch_eval_ranges(st, keys, Ti, E.ranges.as<uint32_t>());
+    launch_tile_ranges_n(st, keys, Ti, ntiles, E.ranges.as<uint32_t>());
   }
   launch_eval_raster(st, E.rec_s.as<float4>(), vals, E.ranges.as<uint32_t>(), C, bg, color, color8, depth,
                      depth_zero);

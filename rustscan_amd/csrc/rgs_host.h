@@ -83,10 +83,10 @@ struct StepPlanKey {
 // between calls: evaluation renders one frame per call).
 struct EvalScratch {
   DevBuf rec, box, live, ntl, dkey, pos, ck, cv, kb, vb, offs, rec_s, box_s, ik0, iv0, ik1, iv1, ranges, sort, scan,
-      counts, parts, sums, img8;
+      counts, parts, sums, img8, brank;
   void release_all() {
     DevBuf* b[] = {&rec, &box, &live, &ntl, &dkey, &pos, &ck, &cv, &kb, &vb, &offs, &rec_s, &box_s,
-                   &ik0, &iv0, &ik1, &iv1, &ranges, &sort, &scan, &counts, &parts, &sums, &img8};
+                   &ik0, &iv0, &ik1, &iv1, &ranges, &sort, &scan, &counts, &parts, &sums, &img8, &brank};
     for (DevBuf* x : b) {
       if (x->p) hipFree(x->p);
       x->p = nullptr;
