@@ -281,7 +281,10 @@ __global__ __launch_bounds__(256) void k_eval_expand(const uint32_t* __restrict_
 // records at a time, with a mask of the blocks each record's 3-sigma ellipse
 // can reach (block_touch_mask on the Mahalanobis form, bound d^2/2 < 4.5:
 // every pixel centre of an unmarked block provably has d^2 >= 9, where the
-// reference skips the splat).  A wave walks only the records that reach its
+// reference skips the splat).  The pixel's d^2 below rounds each of its
+// three terms at most five times and the mask's coefficients a, b, c once
+// each: together within 1e-6 (a dx^2 + c dy^2) of the exact form, inside the
+// mask's 2e-6 margin.  A wave walks only the records that reach its
 // block, in list order, with the reference's per-pixel arithmetic.  A pixel
 // stops once its alpha reaches 0.999 (the reference skips it for every later
 // splat, and alpha never decreases); a wave stops when all its pixels have,
