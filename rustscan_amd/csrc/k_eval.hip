@@ -19,7 +19,7 @@
 //                        live Gaussians, ties by index (stable)
 //   k_eval_gather        records and tile boxes into depth order
 //   k_eval_block_ranks, k_eval_expand  one thread per (Gaussian, tile) instance
-//   (radix sort of tile ids, stable)  per-tile lists in depth order
+//   (radix sort of binning-tile ids, stable)  per-tile lists in depth order
 //   (tile ranges)        [start, end) of every tile's list
 //   k_eval_raster        one 16x16 workgroup per tile, an 8x8 block per wave
 // Metrics (evaluation/core.rs:238-319): k_eval_metrics sums the float32
@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void k_eval_project(const float* __restrict__ 
       txmax = (xmax + kEvalTile - 1) / kEvalTile;
       tymin = ymin / kEvalTile;
       tymax = (ymax + kEvalTile - 1) / kEvalTile;
-      if (txmin < txmax && tymin < tymax) count = (txmax - txmin) * (tymax - tymin);
+      // instances: the binning tiles (2x2 reference tiles) the box touches
+      if (txmin < txmax && tymin < tymax) count = (((txmax + 1) >> 1) - (txmin >> 1)) * (((tymax + 1) >> 1) - (tymin >> 1));
     }
     live[g] = count ? 1u : 0u;
     ntl[g] = count;
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(256) void k_eval_gather(const uint32_t* __restrict_
 
 // Instance i belongs to the depth rank r with offs[r] <= i < offs[r + 1]
 // (every live Gaussian has at least one tile, so offs is strictly
-// increasing); its k-th tile in row-major box order.  Expansion works on
+// increasing); its k-th binning tile in row-major order over the binning
+// tiles its reference-tile box touches.  Expansion works on
 // 256-instance blocks; k_eval_block_ranks writes the rank of each block's
 // first instance (rank r writes it for the blocks that start in its range).
 __global__ __launch_bounds__(256) void k_eval_block_ranks(const uint32_t* __restrict__ offs, uint32_t V, uint32_t T,
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(256) void k_eval_block_ranks(const uint32_t* __rest
 __global__ __launch_bounds__(256) void k_eval_expand(const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ brank,
                                                      const uint2* __restrict__ box_s, uint32_t V, uint32_t T,
-                                                     uint32_t ntx, uint32_t* __restrict__ keys,
+                                                     uint32_t nbx, uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
   __shared__ uint32_t s_off[256];
   const uint32_t r0 = brank[blockIdx.x];
@@ -268,17 +270,20 @@ __global__ __launch_bounds__(256) void k_eval_expand(const uint32_t* __restrict_
   const uint32_t lo = r0 + q;
   const uint32_t k = i - s_off[q];
   const uint2 b = box_s[lo];
-  const uint32_t txmin = b.x & 0xffffu, txmax = b.x >> 16, tymin = b.y & 0xffffu;
-  const uint32_t wt = txmax - txmin;  // >= 1: the box of a live Gaussian covers a tile
+  const uint32_t bx0 = (b.x & 0xffffu) >> 1, bx1 = ((b.x >> 16) + 1) >> 1, by0 = (b.y & 0xffffu) >> 1;
+  const uint32_t wt = bx1 - bx0;  // >= 1: the box of a live Gaussian covers a tile
   uint32_t row, col;
   bbox_rc(k, wt, __builtin_amdgcn_rcpf((float)wt), &row, &col);  // k < 2^20: exact
-  keys[i] = (tymin + row) * ntx + txmin + col;
+  keys[i] = (by0 + row) * nbx + bx0 + col;
   vals[i] = lo;
 }
 
 // One workgroup per 16x16 tile, one pixel per thread; each 64-lane wave owns
-// an 8x8 block of the tile.  The tile's list is staged through LDS 256
-// records at a time, with a mask of the blocks each record's 3-sigma ellipse
+// an 8x8 block of the tile.  The list of the tile's binning tile is staged
+// through LDS 256 records at a time; a record whose reference-tile box does
+// not hold this tile is not in the reference's list for it and is masked out
+// (the filter keeps the list's depth order), the others get a mask of the
+// blocks their 3-sigma ellipse
 // can reach (block_touch_mask on the Mahalanobis form, bound d^2/2 < 4.5:
 // every pixel centre of an unmarked block provably has d^2 >= 9, where the
 // reference skips the splat).  The pixel's d^2 below rounds each of its
@@ -289,9 +294,10 @@ __global__ __launch_bounds__(256) void k_eval_expand(const uint32_t* __restrict_
 // stops once its alpha reaches 0.999 (the reference skips it for every later
 // splat, and alpha never decreases); a wave stops when all its pixels have,
 // the workgroup when every pixel has.
-__global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ rec_s, const uint32_t* __restrict__ vals,
+__global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ rec_s, const uint2* __restrict__ box_s,
+                                                     const uint32_t* __restrict__ vals,
                                                      const uint32_t* __restrict__ ranges, uint32_t W, uint32_t H,
-                                                     uint32_t ntx, float bg0, float bg1, float bg2,
+                                                     uint32_t ntx, uint32_t nbx, float bg0, float bg1, float bg2,
                                                      float* __restrict__ color, uint8_t* __restrict__ color8,
                                                      float* __restrict__ depth, int depth_zero) {
   __shared__ float4 sa[256], sb[256], sc[256];
@@ -305,20 +311,23 @@ __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ 
   const float fpx = (float)px + 0.5f, fpy = (float)py + 0.5f;
   float alpha = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, dep = FLT_MAX;
   bool done = !inside;
-  const uint32_t start = ranges[2 * (size_t)tile], end = ranges[2 * (size_t)tile + 1];
+  const uint32_t bin = (ty >> 1) * nbx + (tx >> 1);
+  const uint32_t start = ranges[2 * (size_t)bin], end = ranges[2 * (size_t)bin + 1];
   for (uint32_t base = start; base < end; base += 256) {
     if (__syncthreads_count(done ? 0 : 1) == 0) break;
     const uint32_t i = base + threadIdx.x;
     uint32_t bits = 0;
     if (i < end) {
       const uint32_t r = vals[i];
+      const uint2 bx = box_s[r];
       const float4 A = rec_s[3 * (size_t)r], B = rec_s[3 * (size_t)r + 1];
       sa[threadIdx.x] = A;
       sb[threadIdx.x] = B;
       sc[threadIdx.x] = rec_s[3 * (size_t)r + 2];
       // d^2 = a dx^2 + 2 b dx dy + c dy^2 with the conic of {cxx, 2 cxy, cyy, 1/det}
       const float a = B.z * B.w, b = -0.5f * B.y * B.w, c = B.x * B.w;
-      bits = block_touch_mask<kEvalTile>(make_float4(A.x, A.y, 0.5f * a, b), 0.5f * c, 4.5f, tox, toy);
+      const bool held = tx >= (bx.x & 0xffffu) && tx < (bx.x >> 16) && ty >= (bx.y & 0xffffu) && ty < (bx.y >> 16);
+      bits = held ? block_touch_mask<kEvalTile>(make_float4(A.x, A.y, 0.5f * a, b), 0.5f * c, 4.5f, tox, toy) : 0u;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -489,19 +498,19 @@ void launch_eval_gather(hipStream_t s, const uint32_t* order, uint32_t V, const 
 }
 
 // brank: (T + 255) / 256 words.
-void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t ntx,
+void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t nbx,
                         uint32_t* brank, uint32_t* keys, uint32_t* vals) {
   if (T == 0 || V == 0) return;
   k_eval_block_ranks<<<(V + 255) / 256, 256, 0, s>>>(offs, V, T, brank);
-  k_eval_expand<<<(T + 255) / 256, 256, 0, s>>>(offs, brank, box_s, V, T, ntx, keys, vals);
+  k_eval_expand<<<(T + 255) / 256, 256, 0, s>>>(offs, brank, box_s, V, T, nbx, keys, vals);
 }
 
 
-void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint32_t* vals, const uint32_t* ranges,
-                        const EvalCam& C, const float bg[3], float* color, uint8_t* color8, float* depth,
-                        int depth_zero) {
-  k_eval_raster<<<C.ntx * C.nty, 256, 0, s>>>(rec_s, vals, ranges, C.W, C.H, C.ntx, bg[0], bg[1], bg[2], color,
-                                              color8, depth, depth_zero);
+void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint2* box_s, const uint32_t* vals,
+                        const uint32_t* ranges, const EvalCam& C, const float bg[3], float* color, uint8_t* color8,
+                        float* depth, int depth_zero) {
+  k_eval_raster<<<C.ntx * C.nty, 256, 0, s>>>(rec_s, box_s, vals, ranges, C.W, C.H, C.ntx, C.nbx, bg[0], bg[1],
+                                              bg[2], color, color8, depth, depth_zero);
 }
 
 uint32_t eval_metrics_blocks(uint32_t W, uint32_t H) {

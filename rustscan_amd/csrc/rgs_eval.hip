@@ -21,12 +21,12 @@ void launch_eval_compact(hipStream_t s, const uint32_t* live, const uint32_t* po
                          uint32_t* ck, uint32_t* cv);
 void launch_eval_gather(hipStream_t s, const uint32_t* order, uint32_t V, const float4* rec, const uint2* box,
                         float4* rec_s, uint2* box_s);
-void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t ntx,
+void launch_eval_expand(hipStream_t s, const uint32_t* offs, const uint2* box_s, uint32_t V, uint32_t T, uint32_t nbx,
                         uint32_t* brank, uint32_t* keys, uint32_t* vals);
 void launch_tile_ranges_n(hipStream_t s, const uint32_t* keys, uint32_t n, uint32_t num_tiles, uint32_t* ranges);
-void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint32_t* vals, const uint32_t* ranges,
-                        const EvalCam& C, const float bg[3], float* color, uint8_t* color8, float* depth,
-                        int depth_zero);
+void launch_eval_raster(hipStream_t s, const float4* rec_s, const uint2* box_s, const uint32_t* vals,
+                        const uint32_t* ranges, const EvalCam& C, const float bg[3], float* color, uint8_t* color8,
+                        float* depth, int depth_zero);
 uint32_t eval_metrics_blocks(uint32_t W, uint32_t H);
 void launch_eval_metrics(hipStream_t s, const float* rend, const uint8_t* rend8, const float* tgt, uint32_t W,
                          uint32_t H, double* parts, double* out);
@@ -72,6 +72,8 @@ EvalCam eval_camera(const rgs_camera* cam, float cov_blur) {
   C.H = cam->height;
   C.ntx = (C.W + kEvalTile - 1) / kEvalTile;
   C.nty = (C.H + kEvalTile - 1) / kEvalTile;
+  C.nbx = (C.ntx + 1) / 2;
+  C.nby = (C.nty + 1) / 2;
   return C;
 }
 
@@ -86,9 +88,9 @@ int eval_render(rgs_ctx* c, const rgs_splats* s, const rgs_camera* cam, const fl
   const EvalCam C = eval_camera(cam, cov_blur);
   const float bg[3] = {background ? background[0] : 0.0f, background ? background[1] : 0.0f,
                        background ? background[2] : 0.0f};
-  const uint32_t ntiles = C.ntx * C.nty;
+  const uint32_t nbins = C.nbx * C.nby;  // the per-tile lists are kept per binning tile
   const uint32_t n = (uint32_t)s->n;
-  rc |= ensure(c, E.ranges, (size_t)ntiles * 8);
+  rc |= ensure(c, E.ranges, (size_t)nbins * 8);
   rc |= ensure(c, E.counts, 16);
   if (n) {
     rc |= ensure(c, E.rec, (size_t)n * 48);
@@ -102,7 +104,7 @@ int eval_render(rgs_ctx* c, const rgs_splats* s, const rgs_camera* cam, const fl
     rc |= ensure(c, E.scan, 4096 * 4);
   }
   if (rc) return RGS_ERR_HIP;
-  launch_fill_u32(st, E.ranges.as<uint32_t>(), 0u, (size_t)ntiles * 2);
+  launch_fill_u32(st, E.ranges.as<uint32_t>(), 0u, (size_t)nbins * 2);
   uint32_t V = 0;
   uint64_t T = 0;
   if (n) {
@@ -149,20 +151,20 @@ int eval_render(rgs_ctx* c, const rgs_splats* s, const rgs_camera* cam, const fl
              E.scan.as<uint32_t>());
     launch_eval_gather(st, order, V, E.rec.as<float4>(), E.box.as<uint2>(), E.rec_s.as<float4>(),
                        E.box_s.as<uint2>());
-    launch_eval_expand(st, E.offs.as<uint32_t>(), E.box_s.as<uint2>(), V, Ti, C.ntx, E.brank.as<uint32_t>(),
+    launch_eval_expand(st, E.offs.as<uint32_t>(), E.box_s.as<uint2>(), V, Ti, C.nbx, E.brank.as<uint32_t>(),
                        E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>());
     // per-tile lists: stable sort by tile id keeps the depth order inside a
     // tile (two packed passes when the tile id and the record index fit one
     // word, as in the training step's tile sort)
     const int w2 = radix_sort_pairs_packed(st, E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), E.ik1.as<uint32_t>(),
                                            E.iv1.as<uint32_t>(), E.ik0.as<uint32_t>(), E.iv0.as<uint32_t>(), nullptr,
-                                           Ti, Ti, (int)ceil_log2(ntiles), std::max(1, (int)ceil_log2(V)), sort);
+                                           Ti, Ti, (int)ceil_log2(nbins), std::max(1, (int)ceil_log2(V)), sort);
     const uint32_t* keys = w2 == 1 ? E.ik1.as<uint32_t>() : E.ik0.as<uint32_t>();
     vals = w2 == 1 ? E.iv1.as<uint32_t>() : E.iv0.as<uint32_t>();
-    launch_tile_ranges_n(st, keys, Ti, ntiles, E.ranges.as<uint32_t>());
+    launch_tile_ranges_n(st, keys, Ti, nbins, E.ranges.as<uint32_t>());
   }
-  launch_eval_raster(st, E.rec_s.as<float4>(), vals, E.ranges.as<uint32_t>(), C, bg, color, color8, depth,
-                     depth_zero);
+  launch_eval_raster(st, E.rec_s.as<float4>(), E.box_s.as<uint2>(), vals, E.ranges.as<uint32_t>(), C, bg, color,
+                     color8, depth, depth_zero);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
   return RGS_OK;

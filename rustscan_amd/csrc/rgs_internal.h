@@ -45,9 +45,13 @@ struct EvalCam {
   float pos[3];  // camera centre in world space
   float fx, fy, cx, cy;
   float blur;
-  uint32_t W, H, ntx, nty;
+  uint32_t W, H, ntx, nty;  // image size; reference tiles across and down
+  uint32_t nbx, nby;        // binning tiles (2x2 reference tiles) across and down
 };
 constexpr uint32_t kEvalTile = 16;
+// The evaluation renderer's lists are binned on tiles of 2x2 reference tiles
+// (32 pixels); a 16-pixel tile keeps the entries whose box holds it.
+constexpr uint32_t kEvalBin = 2 * kEvalTile;
 constexpr uint32_t kEvalMaxSide = 16384;  // image side limit (tile ranges pack in 16 bits)
 constexpr int kEvalSums = 8;               // double sums per block of the metrics kernel
 
