@@ -401,8 +401,11 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_down(const uint32_t* __re
 
 // ------------------------------------------------------------- emission
 // map_gaussian_to_intersects.wgsl:15-49: for each visible instance in depth
-// order, write (global tile id, instance) for every tile of its bbox that
-// passes the tile test, at offs[i] + k, in row-major bbox order.
+// order, write (global tile id, Gaussian index within its view) for every
+// tile of its bbox that passes the tile test, at offs[i] + k, in row-major
+// bbox order.  The tile id fixes the view, so the instance is
+// view * n + index; the shorter value lets the packed tile sort carry it
+// with the tile id in one word for larger batches.
 //  * Bboxes of at most 64 tiles carry their pass mask from the projection
 //    pass; the 256 instances of a workgroup own one contiguous output range,
 //    which is assembled in LDS in windows of kEmitStage slots and written out
@@ -434,13 +437,15 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
   const uint32_t i = i0 + threadIdx.x;
   const bool live = i < V;
-  uint32_t inst = 0, pos = 0, bb[4] = {0u, 0u, 0u, 0u}, tbase = 0, tx_stride = 0;
+  uint32_t inst = 0, gid = 0, pos = 0, bb[4] = {0u, 0u, 0u, 0u}, tbase = 0, tx_stride = 0;
   float mx = 0.f, my = 0.f, a = 0.f, b = 0.f, c = 0.f, thr = 0.f;
   uint64_t mask = 0;
   bool big = false;
   if (live) {
     inst = order[i];
-    const ViewDev& VD = views[inst / n_per_view];
+    const uint32_t view = inst / n_per_view;
+    gid = inst - view * n_per_view;
+    const ViewDev& VD = views[view];
     tbase = VD.tile_base;
     tx_stride = VD.tiles_x;
     const ProjSplat s = proj[inst];
@@ -467,7 +472,7 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
     for (int e = 0; e < 4; ++e) jb[e] = (uint32_t)__shfl((int)bb[e], j, 64);
     const float jx = __shfl(mx, j, 64), jy = __shfl(my, j, 64), ja = __shfl(a, j, 64), jbb = __shfl(b, j, 64),
                 jc = __shfl(c, j, 64), jt = __shfl(thr, j, 64);
-    const uint32_t jinst = (uint32_t)__shfl((int)inst, j, 64), jpos = (uint32_t)__shfl((int)pos, j, 64);
+    const uint32_t jgid = (uint32_t)__shfl((int)gid, j, 64), jpos = (uint32_t)__shfl((int)pos, j, 64);
     const uint32_t jtb = (uint32_t)__shfl((int)tbase, j, 64), jts = (uint32_t)__shfl((int)tx_stride, j, 64);
     const uint32_t bw = jb[2] - jb[0], nb = (jb[3] - jb[1]) * bw;
     const float inv = 1.0f / (float)bw;
@@ -485,7 +490,7 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
         const uint32_t p = run + (uint32_t)__popcll(m & lt);
         if (p < cap) {
           okeys[p] = jtb + (jb[0] + col) + (jb[1] + row) * jts;
-          ovals[p] = jinst;
+          ovals[p] = jgid;
         } else {
           over = true;
         }
@@ -512,7 +517,7 @@ __global__ __launch_bounds__(256) void k_emit(const ProjSplat* __restrict__ proj
           uint32_t row, col;
           bbox_rc(k, bw, inv, &row, &col);
           sk[p - lo] = tbase + (bb[0] + col) + (bb[1] + row) * tx_stride;
-          sv[p - lo] = inst;
+          sv[p - lo] = gid;
         }
         ++p;
       }

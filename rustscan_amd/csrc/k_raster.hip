@@ -266,12 +266,12 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
         const uint32_t j = j0 + tid;
         uint32_t bits = 0;
         if (j < cnt) {
-          const uint32_t inst = A.isect_vals[b + j];
-          const ProjSplat q = A.proj[inst];
+          const uint32_t gid = A.isect_vals[b + j];
+          const ProjSplat q = A.proj[(uint32_t)v * A.n + gid];
           rec[j].p0 = q.p0;
           rec[j].p1 = make_float4(q.p1.x, q.p1.y, fmaxf_(q.p1.z, 0.0f), fmaxf_(q.p1.w, 0.0f));
           rec[j].p2 = make_float4(fmaxf_(q.p2.x, 0.0f), q.p2.y + kSkipMargin, 0.0f, 0.0f);
-          sg[j] = inst - (uint32_t)v * A.n;
+          sg[j] = gid;
           bits = A.cull ? block_touch_mask<RW>(q.p0, q.p1.x, q.p2.y + kSkipMargin, tox, toy)
                         : (1u << ((RW / 8) * (RW / 8))) - 1u;
         }
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         const uint32_t j = j0 + tid;
         uint32_t bits = 0;
         if (j < cnt) {
-          const uint32_t inst = A.isect_vals[b + j];
+          const uint32_t inst = (uint32_t)v * A.n + A.isect_vals[b + j];
           const ProjSplat q = A.proj[inst];
           rec[j].p0 = q.p0;
           rec[j].p1 = make_float4(q.p1.x, q.p1.y, clamp_colour(q.p1.z), clamp_colour(q.p1.w));

@@ -316,8 +316,8 @@ int enqueue_forward(rgs_ctx* c, const rgs_splats* s, const ViewSet& vs, int v0, 
     else
       wt = radix_sort_pairs_packed(st, w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(), w.ik1.as<uint32_t>(),
                                    w.iv1.as<uint32_t>(), w.ik0.as<uint32_t>(), w.iv0.as<uint32_t>(),
-                                   counters + kCntIsect, 0, (uint32_t)w.cap_isect, bits, inst_bits,
-                                   w.scratch.as<uint32_t>());
+                                   counters + kCntIsect, 0, (uint32_t)w.cap_isect, bits,
+                                   std::max(1, (int)ceil_log2(n)), w.scratch.as<uint32_t>());
   }
   uint32_t* skeys = wt == 1 ? w.ik1.as<uint32_t>() : w.ik0.as<uint32_t>();
   uint32_t* svals = wt == 1 ? w.iv1.as<uint32_t>() : w.iv0.as<uint32_t>();
@@ -1478,7 +1478,7 @@ int rgs_debug_get(rgs_ctx* c, int what, int view, void* dst, int64_t* count) {
         res.swap(rg);
       } else {
         for (uint32_t i = lo; i < hi; ++i)
-          res.push_back(what == RGS_DEBUG_ISECT_TILES ? keys[i] - V.tile_base : vals[i] - view * n);
+          res.push_back(what == RGS_DEBUG_ISECT_TILES ? keys[i] - V.tile_base : vals[i]);
       }
       *count = (int64_t)res.size();
       if (dst && !res.empty()) HIPCHK(c, hipMemcpy(dst, res.data(), 4 * res.size(), hipMemcpyDefault));

@@ -147,7 +147,7 @@ struct LossCoef {
 // [tile_first, tile_first + tile_count).
 struct RasterArgs {
   const ProjSplat* proj;
-  const uint32_t* isect_vals;  // sorted instance ids
+  const uint32_t* isect_vals;  // Gaussian index of each sorted intersection (instance = view * n + index)
   const uint32_t* ranges;      // [2 * tiles] (start, end) into isect_vals
   const ViewDev* views;
   int v0, nv;
