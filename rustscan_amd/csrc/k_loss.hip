@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ vi
   }
   __syncthreads();
   for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(fast)
     float (*sx)[kLS + 1] = sxa[c];
     float (*sy)[kLS + 1] = sya[c];
     for (int i = tid; i < kLS * kLB; i += 256) {
@@ -172,17 +173,18 @@ __global__ __launch_bounds__(kLossParts) void k_loss_finish(float* __restrict__ 
 
 // Adjoint of one clamped 11-tap pass at position q of a line of n samples:
 // taps from p in [q-5, q+5] that land on q, plus (at the two ends) the taps
-// that the clamp folded onto q.  at(p) returns the gradient at sample p.
+// that the clamp folded onto q.  at(p) returns the gradient at sample p, and
+// 0 for p outside [0, n) (the callers stage zeros there).
 // (Tap indices are compile-time constants: the coefficients are kernel
 // arguments, which a run-time index would have to fetch from memory.)
+// The loss and its gradient are compared within a relative tolerance, so
+// the blur sums here and in k_loss_fwd use fused multiply-adds.
 template <class F>
 __device__ __forceinline__ float adjoint_tap(const LossCoef& L, int q, int n, F at) {
+#pragma clang fp contract(fast)
   float acc = 0.0f;
 #pragma unroll
-  for (int t = 0; t < 11; ++t) {  // sample p = q + 5 - t reaches q through tap t
-    const int p = q + kLR - t;
-    if (p >= 0 && p < n) acc += at(p) * L.k[t];
-  }
+  for (int t = 0; t < 11; ++t) acc += at(q + kLR - t) * L.k[t];  // sample q + 5 - t reaches q through tap t
   if (q == 0) {  // taps of samples 0..4 that fell below the image
 #pragma unroll
     for (int p = 0; p < kLR; ++p)
