@@ -120,10 +120,12 @@ __global__ __launch_bounds__(256) void k_loss_fwd(const ViewDev* __restrict__ vi
       const float a1 = mxy * 2.0f + L.c1, a2 = sxy * 2.0f + L.c2;
       const float b1r = (mxx + myy) + L.c1, b2r = (sxx + syy) + L.c2;
       const float b1 = fmaxf_(b1r, 1e-8f), b2 = fmaxf_(b2r, 1e-8f);
-      const float s = (a1 * a2) / (b1 * b2);
+      // one division: the quotients below are products with 1 / (b1 b2)
+      const float inv = 1.0f / (b1 * b2);
+      const float s = (a1 * a2) * inv;
       // partial derivatives of s (clamp_min passes no gradient where it clamps)
-      const float d_a1 = a2 / (b1 * b2), d_a2 = a1 / (b1 * b2);
-      const float d_b1 = b1r > 1e-8f ? -s / b1 : 0.0f, d_b2 = b2r > 1e-8f ? -s / b2 : 0.0f;
+      const float d_a1 = a2 * inv, d_a2 = a1 * inv;
+      const float d_b1 = b1r > 1e-8f ? -s * (b2 * inv) : 0.0f, d_b2 = b2r > 1e-8f ? -s * (b1 * inv) : 0.0f;
       const float g_mu = 2.0f * mu_y * (d_a1 - d_a2) + 2.0f * mu_x * (d_b1 - d_b2);
       const float scale = -L.ssim / N;
       float* gm = V.lgrad + (size_t)(3 * c) * np + (size_t)gy * W + gx;
