@@ -115,6 +115,59 @@ def run_eval_render(ctx, state, cam, target, reps: int = 10) -> dict:
             "psnr_db": metrics["psnr_db"], "_images": images}
 
 
+def run_topology_and_files(ctx, opt, state, cams, targets, steps: int = 10) -> dict:
+    """One device topology step (densify / prune / reset) on the 1,048,576-
+    Gaussian C4 scene after `steps` batched steps that gather the
+    densification statistics, then a save and load of the resulting store as
+    the reference's ASCII PLY; wall clock per synchronous call.  RustGS's
+    defaults except: refine_every 1 (the schedule applies at once), growth
+    and split gradient thresholds 1e-6 (ten steps of a synthetic scene
+    gather less than the defaults' 1.4e-4 of a trained one) and threshold
+    pruning at opacity 0.05, so that the step clones, splits and prunes."""
+    import os
+    import shutil
+    import tempfile
+
+    from rustscan_amd import scene_io
+    from rustscan_amd.rasterizer import DeviceSplats, TopologyConfig, Trainer
+
+    sp = DeviceSplats(ctx, state)
+    tr = Trainer(ctx, opt, cov_blur=0.3, loss_weight=1.0 / len(cams), densify=RGS_DENSIFY_ACCUMULATE)
+    for _ in range(steps):
+        tr.train_step(sp, cams, targets, batched=True, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cfg = TopologyConfig(refine_every=1, growth_grad_threshold=1e-6, split_grad_threshold=1e-6, prune_mode=0,
+                         prune_opacity_threshold=0.05)
+    res = sp.topology_step(cfg, 100, len(cams))
+    ctx.sync()
+    t_topo = time.perf_counter() - t0
+    topo = {"ms": round(1e3 * t_topo, 3), "before": int(res["before"]), "after": int(res["after"]),
+            "added": int(res["added"]), "pruned": int(res["pruned"]),
+            "clone_candidates": int(res["clone_candidates"]), "split_candidates": int(res["split_candidates"])}
+    log(f"topology step: {topo}")
+    d = tempfile.mkdtemp(prefix="rgs_bench_")
+    try:
+        path = os.path.join(d, "scene.ply")
+        t0 = time.perf_counter()
+        scene_io.save_device_splats(ctx, sp, path)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        loaded, _ = scene_io.load_device_splats(ctx, path)
+        ctx.sync()
+        t_load = time.perf_counter() - t0
+        files = {"format": "PLY, ASCII (the reference's writer)", "gaussians": int(sp.n), "ply_bytes": int(size), "save_ms": round(1e3 * t_save, 1),
+                 "load_ms": round(1e3 * t_load, 1), "save_mb_per_s": round(size / t_save / 1e6, 1),
+                 "load_mb_per_s": round(size / t_load / 1e6, 1)}
+        loaded.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"scene file (ASCII PLY): {files}")
+    sp.close()
+    return {"topology_step": topo, "scene_file": files}
+
+
 def run_config3(ctx, opt, init, gt, steps: int = 200) -> dict:
     """The reference's training semantics: one 1280x720 view and one Adam
     update per step, cycling over 32 ring cameras, targets rendered from the
@@ -556,6 +609,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_tiled_renderer": cpu_render,
         "evaluation_render": eval_render,
+        "topology_and_files": run_topology_and_files(ctx, opt, state, cams, targets),
         "config3": None if args.no_config3 else run_config3(ctx, opt, init, gt),
         "config5": None if args.no_config5 else run_config5(args, ctx, opt),
         "device": dev_name,
