@@ -318,16 +318,20 @@ __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ 
     const uint32_t i = base + threadIdx.x;
     uint32_t bits = 0;
     if (i < end) {
+      // the box first: only records held by this tile are loaded and staged
+      // (a slot left unstaged has an empty mask and is never read)
       const uint32_t r = vals[i];
       const uint2 bx = box_s[r];
-      const float4 A = rec_s[3 * (size_t)r], B = rec_s[3 * (size_t)r + 1];
-      sa[threadIdx.x] = A;
-      sb[threadIdx.x] = B;
-      sc[threadIdx.x] = rec_s[3 * (size_t)r + 2];
-      // d^2 = a dx^2 + 2 b dx dy + c dy^2 with the conic of {cxx, 2 cxy, cyy, 1/det}
-      const float a = B.z * B.w, b = -0.5f * B.y * B.w, c = B.x * B.w;
       const bool held = tx >= (bx.x & 0xffffu) && tx < (bx.x >> 16) && ty >= (bx.y & 0xffffu) && ty < (bx.y >> 16);
-      bits = held ? block_touch_mask<kEvalTile>(make_float4(A.x, A.y, 0.5f * a, b), 0.5f * c, 4.5f, tox, toy) : 0u;
+      if (held) {
+        const float4 A = rec_s[3 * (size_t)r], B = rec_s[3 * (size_t)r + 1];
+        sa[threadIdx.x] = A;
+        sb[threadIdx.x] = B;
+        sc[threadIdx.x] = rec_s[3 * (size_t)r + 2];
+        // d^2 = a dx^2 + 2 b dx dy + c dy^2 with the conic of {cxx, 2 cxy, cyy, 1/det}
+        const float a = B.z * B.w, b = -0.5f * B.y * B.w, c = B.x * B.w;
+        bits = block_touch_mask<kEvalTile>(make_float4(A.x, A.y, 0.5f * a, b), 0.5f * c, 4.5f, tox, toy);
+      }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
