@@ -1,10 +1,10 @@
 """The benchmark configurations at full size through the C ABI.
 
 * 8-view batched step (1,048,576 Gaussians, SH3, 8 views of uneven size) at
-  the configuration the bench ships (32-pixel tiles, batch depth 256,
-  persistent launch, HIP graph): images within 1e-4 of the oracle, each
-  gradient group within 1e-3 (rel-L2) of the oracle's backward summed over
-  the views.
+  the configurations the bench's autotuner chooses between (32-pixel tiles,
+  batch depth 128 and 256, persistent launch, HIP graph): images within 1e-4
+  of the oracle, each gradient group within 1e-3 (rel-L2) of the oracle's
+  backward summed over the views.
 * Single-view training (1,048,576 Gaussians, 32 ring views at 1280x720, one
   view and one Adam update per step, graph replayed): the first step's
   gradients equal the oracle's, 40 replayed steps lower the L1 loss over the
@@ -43,39 +43,44 @@ def test_config4_full_step_at_shipped_config(hip_ctx, orc):
     tg = [scene.Rng(200 + i).uniform(c.width * c.height * 3).reshape(c.height, c.width, 3).astype(np.float32)
           for i, c in enumerate(cams)]
     tgt = [torch.from_numpy(t).cuda() for t in tg]
-    hip_ctx.set_raster(32, 256, True)
+    w = 1.0 / len(cams)
+    runs = {}
     try:
-        ds = DeviceSplats(hip_ctx, init)
-        skipped0 = hip_ctx.skipped_steps()  # the context is shared with tests that skip on purpose
-        say("config4: 1,048,576 Gaussians, 8 views, tile 32 batch 256 persistent, graph")
-        imgs = [im.cpu().numpy() for im in render_forward(hip_ctx, ds, cams)]
-        w = 1.0 / len(cams)
-        tr = Trainer(hip_ctx, training_optimizer(3, **LRS), loss_weight=w)
-        losses = tr.train_step(ds, cams, tgt, batched=True, graph=True, read_loss=True)
-        vt, vs, vo, _ = ds.grads()
-        assert hip_ctx.skipped_steps() == skipped0
-        rvt = np.zeros_like(vt)
-        rvs = np.zeros_like(vs)
-        rvo = np.zeros_like(vo)
-        worst_img = 0.0
-        for i, (cam, t) in enumerate(zip(cams, tg)):
-            r = orc.Render(init, cam, tile_width=32)
-            img = r.image()
-            worst_img = max(worst_img, float(np.abs(imgs[i] - img).max()))
-            loss, v = orc.l1_loss_grad(img, t, weight=w)
-            np.testing.assert_allclose(losses[i], loss, rtol=1e-5)
-            g = r.backward(v, 0)
-            rvt += g["v_transforms"]
-            rvs += g["v_sh"]
-            rvo += g["v_raw_opac"]
-            del r
-        errs = group_errors(vt, vs, vo, rvt, rvs, rvo)
-        say(f"  image max err {worst_img:.2e}; grads " + " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
-        assert worst_img <= 1e-4
-        assert max(errs.values()) <= 1e-3
-        ds.close()
+        for batch in (128, 256):
+            hip_ctx.set_raster(32, batch, True)
+            ds = DeviceSplats(hip_ctx, init)
+            skipped0 = hip_ctx.skipped_steps()  # the context is shared with tests that skip on purpose
+            say(f"config4: 1,048,576 Gaussians, 8 views, tile 32 batch {batch} persistent, graph")
+            imgs = [im.cpu().numpy() for im in render_forward(hip_ctx, ds, cams)]
+            tr = Trainer(hip_ctx, training_optimizer(3, **LRS), loss_weight=w)
+            losses = tr.train_step(ds, cams, tgt, batched=True, graph=True, read_loss=True)
+            runs[batch] = (imgs, losses, ds.grads())
+            assert hip_ctx.skipped_steps() == skipped0
+            ds.close()
     finally:
         hip_ctx.set_raster(16, 256, True)
+    # the oracle's images and gradients depend on the tile width only
+    rvt = rvs = rvo = None
+    ref_imgs, ref_losses = [], []
+    for cam, t in zip(cams, tg):
+        r = orc.Render(init, cam, tile_width=32)
+        img = r.image()
+        loss, v = orc.l1_loss_grad(img, t, weight=w)
+        g = r.backward(v, 0)
+        rvt = g["v_transforms"] if rvt is None else rvt + g["v_transforms"]
+        rvs = g["v_sh"] if rvs is None else rvs + g["v_sh"]
+        rvo = g["v_raw_opac"] if rvo is None else rvo + g["v_raw_opac"]
+        ref_imgs.append(img)
+        ref_losses.append(loss)
+        del r
+    for batch, (imgs, losses, (vt, vs, vo, _)) in runs.items():
+        worst_img = max(float(np.abs(a - b).max()) for a, b in zip(imgs, ref_imgs))
+        np.testing.assert_allclose(losses, ref_losses, rtol=1e-5)
+        errs = group_errors(vt, vs, vo, rvt, rvs, rvo)
+        say(f"  batch {batch}: image max err {worst_img:.2e}; grads " +
+            " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
+        assert worst_img <= 1e-4
+        assert max(errs.values()) <= 1e-3
 
 
 def test_config3_graph_replayed_training(hip_ctx, orc):
