@@ -4,11 +4,14 @@ lists, tile ranges, projected splats, visibility and the image, exactly), the
 backward (every gradient group within 1e-3 rel-L2 of the oracle's FP64 replay
 of the same forward), and the evaluation render with the CPU renderer's
 semantics (bit for bit against the oracle run with the library's
-exponential).
+exponential).  Batched training steps over 1-8 random views in one
+persistent launch, eager or graph-captured, against the oracle's images,
+losses and FP64 gradient replay summed over the views.
 
-The cases cover 1 to 20,000 Gaussians, SH degrees 0-3, images from 1 pixel to
-700x500, log-scale spreads up to 1.5 (needle-shaped splats), camera distances
-from inside to outside the scene, and every tile width with every batch depth.
+The single-view cases cover 1 to 20,000 Gaussians, SH degrees 0-3, images
+from 1 pixel to 700x500, log-scale spreads up to 1.5 (needle-shaped splats),
+camera distances from inside to outside the scene, and every tile width with
+every batch depth.
 """
 import numpy as np
 import pytest
@@ -16,7 +19,7 @@ import pytest
 from rustscan_amd import scene
 from tests import test_gpu_eval as TE
 from tests import test_gpu_render as TR
-from tests.gpu_util import say
+from tests.gpu_util import rel_l2, say
 
 pytestmark = pytest.mark.gpu
 
@@ -58,3 +61,76 @@ def test_random_scene_vs_oracle(hip_ctx, orc, case):
     col, dep = TE._tiled(hip_ctx, sc, cam)
     rc, rd = orc.tiled_renderer(sc, cam, cov_blur=0.3, gpu_math=True)
     TE._assert_same(col, dep, rc, rd, "evaluation render")
+
+
+LRS = dict(lr_position=1.6e-4, lr_rotation=1e-3, lr_scale=5e-3, lr_opacity=5e-2, lr_sh_dc=2.5e-3, lr_sh_rest=1.25e-4)
+
+
+def _step_cases(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        nv = int(rng.integers(1, 9))
+        out.append(dict(
+            n=int(rng.choice([50, 2000, 20000, 60000])),
+            deg=int(rng.integers(0, 4)),
+            sizes=[(int(rng.integers(8, 500)), int(rng.integers(8, 400))) for _ in range(nv)],
+            tile=int(rng.choice([8, 16, 32])),
+            batch=int(rng.choice([128, 256, 512])),
+            graph=bool(rng.integers(0, 2)),
+            mu=float(rng.uniform(-4.0, -2.0)),
+            sigma=float(rng.uniform(0.1, 1.0)),
+            radius=float(rng.uniform(3.0, 6.0)),
+            seed=int(rng.integers(0, 1 << 30)),
+        ))
+    return out
+
+
+@pytest.mark.parametrize("case", _step_cases(8, 20261204),
+                         ids=lambda c: f"n{c['n']}-sh{c['deg']}-v{len(c['sizes'])}-t{c['tile']}b{c['batch']}"
+                                       f"-{'graph' if c['graph'] else 'eager'}")
+def test_random_batched_step_vs_oracle(hip_ctx, orc, case):
+    """A batched training step over 1-8 random views in one persistent launch
+    (eager or graph-captured): every view's image within 1e-4 of the oracle,
+    every view's loss within 1e-5, and the gradients summed over the views
+    within 1e-3 of the oracle's FP64 replay."""
+    import torch
+
+    from rustscan_amd.rasterizer import DeviceSplats, Trainer, render_forward, training_optimizer
+
+    c = case
+    say(f"random batched step {c}")
+    sc = scene.gaussian_scene(c["n"], seed=c["seed"], sh_degree=c["deg"], log_scale_mu=c["mu"],
+                              log_scale_sigma=c["sigma"])
+    cams = scene.ring_cameras(c["sizes"], radius=c["radius"], seed=c["seed"] + 1)
+    tg = [scene.Rng(c["seed"] + 10 + i).uniform(cam.width * cam.height * 3).reshape(cam.height, cam.width, 3)
+          .astype(np.float32) for i, cam in enumerate(cams)]
+    w = 1.0 / len(cams)
+    hip_ctx.set_raster(c["tile"], c["batch"], True)
+    try:
+        ds = DeviceSplats(hip_ctx, sc)
+        imgs = [im.cpu().numpy() for im in render_forward(hip_ctx, ds, cams)]
+        tr = Trainer(hip_ctx, training_optimizer(c["deg"], **LRS), loss_weight=w)
+        losses = tr.train_step(ds, cams, [torch.from_numpy(t).cuda() for t in tg], batched=True, graph=c["graph"],
+                               read_loss=True)
+        vt, vs, vo, _ = ds.grads()
+        ds.close()
+    finally:
+        hip_ctx.set_raster(16, 256, True)
+    rvt = rvs = rvo = None
+    worst = 0.0
+    for i, (cam, t) in enumerate(zip(cams, tg)):
+        r = orc.Render(sc, cam, tile_width=c["tile"])
+        img = r.image()
+        worst = max(worst, float(np.abs(imgs[i] - img).max()))
+        loss, v = orc.l1_loss_grad(img, t, weight=w)
+        assert abs(losses[i] - loss) <= 1e-5 * max(1.0, abs(loss))
+        g = r.backward_f64(v, mode=0)
+        rvt = g["v_transforms"] if rvt is None else rvt + g["v_transforms"]
+        rvs = g["v_sh"] if rvs is None else rvs + g["v_sh"]
+        rvo = g["v_raw_opac"] if rvo is None else rvo + g["v_raw_opac"]
+    errs = dict(means=rel_l2(vt[:, :3], rvt[:, :3]), quats=rel_l2(vt[:, 3:7], rvt[:, 3:7]),
+                log_scales=rel_l2(vt[:, 7:], rvt[:, 7:]), opacity=rel_l2(vo, rvo), sh=rel_l2(vs, rvs))
+    say(f"  image max err {worst:.2e}; grads " + " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
+    assert worst <= 1e-4
+    assert max(errs.values()) <= 1e-3
