@@ -6,7 +6,9 @@ of the same forward), and the evaluation render with the CPU renderer's
 semantics (bit for bit against the oracle run with the library's
 exponential).  Batched training steps over 1-8 random views in one
 persistent launch, eager or graph-captured, against the oracle's images,
-losses and FP64 gradient replay summed over the views.
+losses and FP64 gradient replay summed over the views.  Device topology
+steps on random splats, statistics, state, schedules and thresholds, against
+the topology oracle bit for bit.
 
 The single-view cases cover 1 to 20,000 Gaussians, SH degrees 0-3, images
 from 1 pixel to 700x500, log-scale spreads up to 1.5 (needle-shaped splats),
@@ -134,3 +136,87 @@ def test_random_batched_step_vs_oracle(hip_ctx, orc, case):
     say(f"  image max err {worst:.2e}; grads " + " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
     assert worst <= 1e-4
     assert max(errs.values()) <= 1e-3
+
+
+TOPOLOGY_KEYS = ["before", "after", "added", "pruned", "clone_candidates", "split_candidates", "growth_candidates",
+                 "prune_candidates", "replacement_count", "extra_growth_count", "selected_count", "mutated",
+                 "adam_reset"]
+
+
+def _topology_cases(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        cfg = dict(
+            iterations=int(rng.choice([30, 1000, 30000])),
+            max_initial_gaussians=int(rng.choice([1, 100, 100000])),
+            refine_every=int(rng.choice([1, 2, 5, 160])),
+            opacity_reset_interval=int(rng.choice([1, 2, 10])),
+            opacity_reset_mode=int(rng.integers(0, 2)),
+            growth_freeze_after_epoch=int(rng.choice([-1, 0, 3])),
+            prune_until_epoch=int(rng.choice([-1, 2, 100])),
+            split_score_mode=int(rng.integers(0, 4)),
+            growth_grad_threshold=float(rng.choice([1e-5, 1.4e-4, 1e-3])),
+            split_grad_threshold=float(rng.choice([1e-5, 1.4e-4, 1e-3])),
+            growth_select_fraction=float(rng.choice([0.0, 0.25, 0.5, 1.0])),
+            opacity_decay=float(rng.choice([0.0, 0.1])),
+            scale_decay=float(rng.choice([0.0, 0.2])),
+            prune_mode=int(rng.integers(0, 3)),
+            prune_min_age=int(rng.choice([0, 5])),
+            prune_invisible_epochs=int(rng.choice([1, 10])),
+            prune_visibility_dry_run=int(rng.integers(0, 2)),
+            prune_opacity_threshold=float(rng.choice([1.0 / 255.0, 0.05, 0.3])),
+            prune_visibility_threshold=float(rng.choice([0.05, 0.3])),
+        )
+        out.append(dict(n=int(rng.choice([1, 2, 50, 1000, 20000, 70000])), deg=int(rng.integers(0, 4)),
+                        iteration=int(rng.integers(1, 40)), frame_count=int(rng.integers(1, 6)),
+                        mu=float(rng.uniform(-5.0, -1.0)), seed=int(rng.integers(0, 1 << 30)), cfg=cfg))
+    return out
+
+
+@pytest.mark.parametrize("case", _topology_cases(16, 20261205),
+                         ids=lambda c: f"n{c['n']}-it{c['iteration']}-fc{c['frame_count']}-re{c['cfg']['refine_every']}"
+                                       f"-pm{c['cfg']['prune_mode']}")
+def test_random_topology_step_vs_oracle(hip_ctx, orc, case):
+    """A device topology step on random splats, densification statistics,
+    topology state, schedule and thresholds: counts, schedule, the rebuilt
+    store and the remapped state equal the topology oracle's bit for bit."""
+    from rustscan_amd.rasterizer import AdamScaled, DeviceSplats, TopologyConfig
+
+    c = case
+    n = c["n"]
+    say(f"random topology step n={n} it={c['iteration']} fc={c['frame_count']} {c['cfg']}")
+    rng = np.random.default_rng(c["seed"])
+    sc = scene.gaussian_scene(n, seed=c["seed"], sh_degree=c["deg"], log_scale_mu=c["mu"])
+    d = np.zeros((10, n), np.float32)
+    d[0] = rng.exponential(3e-4, n)
+    d[1] = rng.exponential(1e-3, n)
+    d[2] = rng.exponential(2e-3, n)
+    d[3] = rng.exponential(1e-2, n)
+    d[4] = rng.uniform(0, 40, n)
+    d[5] = d[4] * rng.uniform(1, 8, n)
+    d[6] = rng.exponential(1e-4, n)
+    d[7] = float(rng.integers(0, 6))
+    d[8] = rng.integers(0, 5, n)
+    d[9] = np.minimum(d[8], rng.integers(0, 5, n))
+    birth = rng.integers(0, 19, n).astype(np.uint32)
+    invis = rng.integers(0, 15, n).astype(np.uint32)
+    ref = orc.topology_step(orc.topology_cfg(**c["cfg"]), sc, d, birth, invis, c["iteration"], c["frame_count"])
+    ds = DeviceSplats(hip_ctx, sc)
+    try:
+        ds.set_densify_stats(d)
+        ds.set_topology_state(birth, invis)
+        AdamScaled.set_steps(hip_ctx, ds, [7, 7, 7])
+        res = ds.topology_step(TopologyConfig(**c["cfg"]), c["iteration"], c["frame_count"])
+        say("  " + " ".join(f"{k}={res[k]}" for k in TOPOLOGY_KEYS[:4]))
+        assert {k: res[k] for k in TOPOLOGY_KEYS} == {k: int(ref[k]) for k in TOPOLOGY_KEYS}
+        assert res["schedule"] == ref["schedule"]
+        got = ds.download()
+        np.testing.assert_array_equal(got.transforms, ref["transforms"])
+        np.testing.assert_array_equal(got.sh, ref["sh"])
+        np.testing.assert_array_equal(got.raw_opac, ref["raw_opac"])
+        b, w = ds.topology_state()
+        np.testing.assert_array_equal(b, ref["birth"])
+        np.testing.assert_array_equal(w, ref["invisible_windows"])
+    finally:
+        ds.close()
