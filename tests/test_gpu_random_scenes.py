@@ -8,7 +8,8 @@ exponential).  Batched training steps over 1-8 random views in one
 persistent launch, eager or graph-captured, against the oracle's images,
 losses and FP64 gradient replay summed over the views.  Device topology
 steps on random splats, statistics, state, schedules and thresholds, against
-the topology oracle bit for bit.
+the topology oracle bit for bit.  The combined training loss on random
+image sizes, loss configurations and weights, against the loss oracle.
 
 The single-view cases cover 1 to 20,000 Gaussians, SH degrees 0-3, images
 from 1 pixel to 700x500, log-scale spreads up to 1.5 (needle-shaped splats),
@@ -220,3 +221,50 @@ def test_random_topology_step_vs_oracle(hip_ctx, orc, case):
         np.testing.assert_array_equal(w, ref["invisible_windows"])
     finally:
         ds.close()
+
+
+LOSS_CONFIGS = [dict(), dict(l1_weight=1.0, ssim_weight=0.0), dict(l1_weight=0.0, ssim_weight=1.0),
+                dict(l1_weight=0.3, ssim_weight=0.5, gradient_weight=0.2),
+                dict(l1_weight=1.0, ssim_weight=0.2, robust_delta=0.2),
+                dict(l1_weight=1.0, ssim_weight=0.2, outlier_threshold=0.3, outlier_weight=0.4)]
+
+
+def _loss_cases(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        h = int(rng.integers(2, 80)) if rng.uniform() < 0.75 else int(rng.integers(80, 400))
+        w = int(rng.integers(2, 80)) if rng.uniform() < 0.75 else int(rng.integers(80, 400))
+        out.append(dict(hw=(h, w), cfg=int(rng.integers(0, len(LOSS_CONFIGS))), weight=float(rng.uniform(0.1, 2.0)),
+                        seed=int(rng.integers(0, 1 << 30))))
+    return out
+
+
+@pytest.mark.parametrize("case", _loss_cases(12, 20261206),
+                         ids=lambda c: f"{c['hw'][0]}x{c['hw'][1]}-cfg{c['cfg']}")
+def test_random_image_loss_vs_oracle(hip_ctx, case):
+    """The combined loss and its image gradient on a random image size, loss
+    configuration and weight: loss within 1e-5 relative and gradient within
+    1e-5 rel-L2 of the loss oracle."""
+    import torch
+
+    from oracle import loss as OL
+    from rustscan_amd.rasterizer import LossConfig, image_loss_grad
+
+    c = case
+    h, w = c["hw"]
+    kw = LOSS_CONFIGS[c["cfg"]]
+    rng = np.random.default_rng(c["seed"])
+    pred = rng.uniform(0, 1, (h, w, 4)).astype(np.float32)
+    tgt = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    loss, v = image_loss_grad(hip_ctx, torch.from_numpy(pred).cuda(), torch.from_numpy(tgt).cuda(), c["weight"],
+                              LossConfig(**kw))
+    cfg_o = OL.LossConfig(**kw)
+    ref_loss = c["weight"] * OL.forward(pred, tgt, cfg_o)
+    ref_g = c["weight"] * OL.grad(pred, tgt, cfg_o)
+    got = v.cpu().numpy()
+    err = rel_l2(got[..., :3], ref_g)
+    say(f"random loss {h}x{w} {kw} weight {c['weight']:.3f}: {loss:.6f} vs {ref_loss:.6f}, grad rel-L2 {err:.2e}")
+    assert np.all(got[..., 3] == 0.0)
+    np.testing.assert_allclose(loss, ref_loss, rtol=1e-5)
+    assert err <= 1e-5
