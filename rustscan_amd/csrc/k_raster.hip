@@ -466,17 +466,21 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
           n_eval += pen == 0.0f ? 1u : 0u;
           ++n_iter;
         }
-        const bool need = sigma + pen <= a2.y;
-        if (!wave_any(need)) return;
+        // The decisions are combined as lane masks: each compare's ballot is
+        // its own result mask, the masks combine on the scalar unit, and the
+        // selects below take them back per lane (inverse ballot) without
+        // materialising the booleans in vector registers.
+        const uint64_t m_need = __builtin_amdgcn_ballot_w64(sigma + pen <= a2.y);
+        if (m_need == 0) return;
         const float gauss = rgs_exp_core_fast(-sigma);
         const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
-        // bitwise, not short-circuit: lane masks combine without branches
-        const bool pass = need & !((sigma < 0.0f) | (alpha < RGS_F(1.0 / 255.0)));
+        const uint64_t m_pass = m_need & ~__builtin_amdgcn_ballot_w64(sigma < 0.0f) &
+                                ~__builtin_amdgcn_ballot_w64(alpha < RGS_F(1.0 / 255.0));
         const float nT = T * (1.0f - alpha);
-        const bool stop = pass & (nT <= RGS_F(1e-4));
-        const bool contrib = pass & !stop;
-        pen = stop ? kInf : pen;
-        const uint64_t m = __builtin_amdgcn_ballot_w64(contrib);
+        const uint64_t m_stop = m_pass & __builtin_amdgcn_ballot_w64(nT <= RGS_F(1e-4));
+        const uint64_t m = m_pass & ~m_stop;
+        const bool contrib = __builtin_amdgcn_inverse_ballot_w64(m);
+        pen = __builtin_amdgcn_inverse_ballot_w64(m_stop) ? kInf : pen;
         if (m == 0) return;
         if (COUNT) {
           n_contrib += contrib ? 1u : 0u;
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         P = P + vis * Q;  // po includes this splat's own colour
         const float v_alpha_all = T * Q + (P - R) * ra;
         // the alpha clamp gates the geometric terms (rasterize_backwards.wgsl:129-143)
-        const bool geo = contrib & (a1.y * gauss <= RGS_F(0.999));
+        const bool geo = __builtin_amdgcn_inverse_ballot_w64(m & __builtin_amdgcn_ballot_w64(a1.y * gauss <= RGS_F(0.999)));
         const float v_alpha = geo ? v_alpha_all : 0.0f;
         const float v_sigma = -alpha * v_alpha;  // alpha is finite on every lane
         const float ca = 2.0f * a0.z, cb = a0.w, cc = a2.z;
