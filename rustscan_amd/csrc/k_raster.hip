@@ -291,7 +291,9 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_fwd(RasterArgs A
         const bool need = sigma + pen <= a2.y;
         if (!wave_any(need)) return;
         if (COUNT) ++n_iter_work;
-        const float alpha = fminf_(RGS_F(0.999), a1.y * rgs_exp_core_fast(-sigma));
+        // v_min_f32: equal to the reference's min for every non-NaN product;
+        // a NaN (only on lanes rejected by the sigma test) gives 0.999
+        const float alpha = __builtin_fminf(RGS_F(0.999), a1.y * rgs_exp_core_fast(-sigma));
         const bool acc = need && sigma >= 0.0f && alpha >= RGS_F(1.0 / 255.0);
         if (acc) {
           if (COUNT) ++n_blend;
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(RasterShape<TW>::NT) void k_raster_bwd(RasterArgs A
         const uint64_t m_need = __builtin_amdgcn_ballot_w64(sigma + pen <= a2.y);
         if (m_need == 0) return;
         const float gauss = rgs_exp_core_fast(-sigma);
-        const float alpha = fminf_(RGS_F(0.999), a1.y * gauss);
+        const float alpha = __builtin_fminf(RGS_F(0.999), a1.y * gauss);  // as in the forward
         const uint64_t m_pass = m_need & ~__builtin_amdgcn_ballot_w64(sigma < 0.0f) &
                                 ~__builtin_amdgcn_ballot_w64(alpha < RGS_F(1.0 / 255.0));
         const float nT = T * (1.0f - alpha);
