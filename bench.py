@@ -269,6 +269,9 @@ def main() -> int:
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the single-view training loop (32 views at 1280x720, one view and one Adam "
                          "update per step, 200 steps)")
+    ap.add_argument("--no-topology", action="store_true",
+                    help="skip the topology step and the PLY save/load (ten more batched steps on a copy of "
+                         "the scene, then one densify/prune step)")
     ap.add_argument("--tile", type=int, default=32)
     ap.add_argument("--batch-depth", type=int, default=128)
     args = ap.parse_args()
@@ -609,7 +612,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_tiled_renderer": cpu_render,
         "evaluation_render": eval_render,
-        "topology_and_files": run_topology_and_files(ctx, opt, state, cams, targets),
+        "topology_and_files": None if args.no_topology else run_topology_and_files(ctx, opt, state, cams, targets),
         "config3": None if args.no_config3 else run_config3(ctx, opt, init, gt),
         "config5": None if args.no_config5 else run_config5(args, ctx, opt),
         "device": dev_name,
