@@ -50,6 +50,16 @@ def test_library_exports_every_declared_symbol(libpath):
         assert hasattr(L, n), n
 
 
+def test_integration_index_lists_the_header():
+    """The entry point index of INTEGRATION.md names every declared function,
+    and nothing else."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    section = text.split("## 5. Entry point index", 1)[1]
+    listed = set(re.findall(r"`(rgs_[a-z0-9_]+)`", section))
+    declared = set(declared_functions())
+    assert listed == declared, (sorted(declared - listed), sorted(listed - declared))
+
+
 def test_python_binding_covers_the_header():
     bound = set(_lib._SIGNATURES)
     declared = set(declared_functions())
