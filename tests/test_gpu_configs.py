@@ -13,10 +13,17 @@
   1920x1080 views in one batched launch: no overflow, binning invariants
   (sorted keys, ranges covering every intersection, counts summing to M),
   and the first view's image equal to the oracle's.
+* The same scene's full batch, 16 views at 1920x1080, in one persistent
+  launch (graph-captured step) at the bench's configuration: loss and every
+  gradient group equal to the per-view launches' (rel-L2 <= 1e-3), no step
+  skipped; the first view's step alone equal to the oracle's gradients (the
+  backward that recomputes tile ranges, at this scene's scale).
 * The autotuner leaves parameters, Adam moments, step counters and the
   skipped-step count as it found them.
 * Adam with weight decay and per-group step counters.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -26,6 +33,12 @@ from tests.gpu_util import rel_l2, say
 pytestmark = pytest.mark.gpu
 
 LRS = dict(lr_position=1.6e-4, lr_rotation=1e-3, lr_scale=5e-3, lr_opacity=5e-2, lr_sh_dc=2.5e-3, lr_sh_rest=1.25e-4)
+
+
+@functools.lru_cache(maxsize=1)
+def config5_scene():
+    """The 4,194,304-Gaussian scene, generated once for the tests that use it."""
+    return scene.config5_scene()
 
 
 def group_errors(vt, vs, vo, rvt, rvs, rvo) -> dict:
@@ -135,7 +148,7 @@ def test_config5_two_views(hip_ctx, orc):
     from rustscan_amd._lib import RGS_DEBUG_ISECT_TILES, RGS_DEBUG_TILE_RANGES
     from rustscan_amd.rasterizer import DeviceSplats, render_forward
 
-    sc = scene.config5_scene()
+    sc = config5_scene()
     cams = scene.config5_cameras()[:2]
     hip_ctx.set_raster(32, 256, True)
     try:
@@ -164,6 +177,51 @@ def test_config5_two_views(hip_ctx, orc):
         ds.close()
     finally:
         hip_ctx.set_raster(16, 256, True)
+
+
+def test_config5_full_batch_gradients(hip_ctx, orc):
+    import torch
+
+    from rustscan_amd.rasterizer import DeviceSplats, Trainer, training_optimizer
+
+    sc = config5_scene()
+    cams = scene.config5_cameras()
+    gen = torch.Generator(device="cuda").manual_seed(10)
+    tgt = [torch.rand((c.height, c.width, 3), device="cuda", generator=gen) for c in cams]
+    w = 1.0 / len(cams)
+    runs = {}
+    hip_ctx.set_raster(32, 128, True)
+    try:
+        skipped0 = hip_ctx.skipped_steps()
+        for batched in (True, False):
+            say(f"config5: 4,194,304 Gaussians, 16 views 1920x1080, {'one batched launch, graph' if batched else 'per-view launches'}")
+            ds = DeviceSplats(hip_ctx, sc)
+            tr = Trainer(hip_ctx, training_optimizer(3, **LRS), loss_weight=w)
+            losses = tr.train_step(ds, cams, tgt, batched=batched, graph=batched, read_loss=True)
+            runs[batched] = (np.asarray(losses), ds.grads()[:3])
+            ds.close()
+        say("config5: view 0 alone")
+        ds = DeviceSplats(hip_ctx, sc)
+        tr = Trainer(hip_ctx, training_optimizer(3, **LRS), loss_weight=w)
+        loss0 = tr.train_step(ds, cams[:1], tgt[:1], batched=True, graph=False, read_loss=True)[0]
+        g0 = ds.grads()[:3]
+        ds.close()
+        assert hip_ctx.skipped_steps() == skipped0
+    finally:
+        hip_ctx.set_raster(16, 256, True)
+    (lb, gb), (lp, gp) = runs[True], runs[False]
+    np.testing.assert_allclose(lb, lp, rtol=1e-5)
+    errs = group_errors(*gb, *gp)
+    say("  batched vs per-view grads " + " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= 1e-3
+    r = orc.Render(sc, cams[0], tile_width=32)
+    loss, v = orc.l1_loss_grad(r.image(), tgt[0].cpu().numpy(), weight=w)
+    g = r.backward(v, 0)
+    del r
+    assert abs(loss0 - loss) <= 1e-5 * abs(loss)
+    errs = group_errors(*g0, g["v_transforms"], g["v_sh"], g["v_raw_opac"])
+    say("  view 0 vs oracle grads " + " ".join(f"{k}={e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= 1e-3
 
 
 def test_autotune_restores_state(hip_ctx):
