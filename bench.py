@@ -235,8 +235,19 @@ def run_config5(args, ctx, opt) -> dict:
         trainer.train_step(splats, cams, targets, batched=True, graph=True)
     ctx.sync()
     dt = (time.perf_counter() - t0) / args.config5_steps
-    counts = ctx.counts(len(cams))
     free, total = torch.cuda.mem_get_info()
+    stages = {}
+    if not args.no_instrument:
+        # where the step's time goes: HIP events around each stage of eager
+        # steps (after the timed graph steps; the parameters keep training)
+        ctx.set_timing(True)
+        ctx.timing(reset=True)
+        for _ in range(args.config5_steps):
+            trainer.train_step(splats, cams, targets, batched=True, graph=False)
+        ctx.sync()
+        stages = {k: round(ms / args.config5_steps, 3) for k, (ms, calls) in ctx.timing(reset=True).items() if calls}
+        ctx.set_timing(False)
+    counts = ctx.counts(len(cams))  # reads the tile ranges and keys back (after the steps)
     out = {
         "workload": "4,194,304 Gaussians SH3 (1% log-scale + ln 20), 16 views 1920x1080, batched, HIP graph",
         "steps_per_s": round(1.0 / dt, 4),
@@ -246,6 +257,7 @@ def run_config5(args, ctx, opt) -> dict:
         "device_memory_in_use_gb": round((total - free) / 1e9, 2),
         "train_mpix_per_s": round(sum(c.width * c.height for c in cams) / dt / 1e6, 1),
         "skipped_steps": ctx.skipped_steps(),
+        "stage_ms_per_step": stages or None,
     }
     log(f"config5: {out}")
     splats.close()
