@@ -128,6 +128,7 @@ class Render:
         self._tf, self._sh, self._op = _f32(splats.transforms), _f32(splats.sh), _f32(splats.raw_opac)
         bg = (c_float * 3)(*background)
         self._c = camera(cam)
+        self._free = L.orc_free  # held here: module globals may be gone when a late __del__ runs at exit
         self.h = L.orc_forward(self.n, splats.sh_degree, self._tf.ctypes.data, self._sh.ctypes.data,
                                self._op.ctypes.data, ctypes.byref(self._c), bg, float(cov_blur), int(tile_width))
         if not self.h:
@@ -138,7 +139,7 @@ class Render:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def image(self) -> np.ndarray:
