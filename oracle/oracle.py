@@ -91,6 +91,8 @@ def lib():
         L.orc_grad_f64.argtypes = [c_int64, c_int, c_void_p, c_void_p, c_void_p, POINTER(OrcCamera), c_void_p,
                                    c_double, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
         L.orc_math_exp.argtypes = [c_void_p, c_void_p, c_int64]
+        L.orc_expf_libm_check.restype = c_int64
+        L.orc_expf_libm_check.argtypes = [c_void_p, c_int]
         L.orc_math_log.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_exp_core.argtypes = [c_void_p, c_void_p, c_int64]
         L.orc_math_exp_core_fast.argtypes = [c_void_p, c_void_p, c_int64]
@@ -368,6 +370,15 @@ def tr_sh_rgb(sh, degree, dirs):
     out = np.zeros((dirs.shape[0], 3), np.float32)
     lib().orc_tr_sh_rgb(dirs.shape[0], degree, sh.ctypes.data, dirs.ctypes.data, out.ctypes.data)
     return out
+
+
+def expf_libm_check(max_out: int = 16):
+    """rgs_expf_libm (the device's exp for the evaluation render) against the
+    C library's expf on every float: (number of differing inputs, the first
+    max_out of those inputs as float32)."""
+    out = np.zeros(max_out, np.uint32)
+    n = lib().orc_expf_libm_check(out.ctypes.data, max_out)
+    return int(n), np.sort(out[:min(n, max_out)]).view(np.float32)
 
 
 def math_exp(x: np.ndarray) -> np.ndarray:

@@ -1587,6 +1587,26 @@ void orc_math_exp_core(const float* x, float* y, int64_t n) {
 void orc_math_exp_core_fast(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_exp_core_fast(x[i]);
 }
+// rgs_expf_libm against this C library's expf over every float input: the
+// number whose result bits differ (two NaNs count as equal); the first
+// max_out of those inputs (as bits) go to out.
+int64_t orc_expf_libm_check(uint32_t* out, int max_out) {
+  int64_t bad = 0;
+  int stored = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (int64_t u = 0; u <= 0xffffffffLL; ++u) {
+    const uint32_t w = (uint32_t)u;
+    const float x = rgs_u2f(w);
+    const float a = std::exp(x), b = rgs_expf_libm(x);
+    if (rgs_f2u(a) != rgs_f2u(b) && !(a != a && b != b)) {
+      ++bad;
+#pragma omp critical
+      if (stored < max_out) out[stored++] = w;
+    }
+  }
+  return bad;
+}
+
 void orc_math_log(const float* x, float* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = rgs_log(x[i]);
 }

@@ -148,6 +148,9 @@ void orc_tr_sh_rgb(int64_t n, int degree, const float* sh, const float* dirs, fl
 
 /* ---- scalar math exported for pinning tests ---- */
 void orc_math_exp(const float* x, float* y, int64_t n);
+/* rgs_expf_libm against the C library's expf on all 2^32 inputs: the count of
+ * inputs with different bits, the first max_out of them (bits) in out. */
+int64_t orc_expf_libm_check(uint32_t* out, int max_out);
 void orc_math_log(const float* x, float* y, int64_t n);
 void orc_math_exp_core(const float* x, float* y, int64_t n);
 void orc_math_exp_core_fast(const float* x, float* y, int64_t n);

@@ -10,8 +10,9 @@
 // alpha-accumulation rule (alpha += w (1 - alpha) while alpha < 0.999,
 // weight cut at 0.001, Mahalanobis cut at 9, depth from the first splat with
 // alpha > 0.01).  These kernels reproduce that arithmetic operation for
-// operation (no contraction; rgs_exp for exp), so that the oracle's
-// restatement with the same exponential matches them bit for bit.
+// operation (no contraction), with the C library's expf bits for exp
+// (rgs_expf_libm), so that they match the renderer with the libm it links,
+// and the oracle's restatement of it, bit for bit.
 //
 // Pipeline (one view):
 //   k_eval_project<DEG>  per Gaussian: colour, projection, culling, tile box
@@ -142,8 +143,8 @@ __global__ __launch_bounds__(256) void k_eval_project(const float* __restrict__ 
                 r12 = 2.0f * (qy * qz - qw * qx);
     const float r20 = 2.0f * (qx * qz - qw * qy), r21 = 2.0f * (qy * qz + qw * qx),
                 r22 = 1.0f - 2.0f * (qx * qx + qy * qy);
-    const float sx = fabsf(rgs_exp(P[7 * (size_t)npad + g])), sy = fabsf(rgs_exp(P[8 * (size_t)npad + g])),
-                sz = fabsf(rgs_exp(P[9 * (size_t)npad + g]));
+    const float sx = fabsf(rgs_expf_libm(P[7 * (size_t)npad + g])), sy = fabsf(rgs_expf_libm(P[8 * (size_t)npad + g])),
+                sz = fabsf(rgs_expf_libm(P[9 * (size_t)npad + g]));
     const float sxx = sx * sx, syy = sy * sy, szz = sz * sz;
     const float cw00 = r00 * r00 * sxx + r01 * r01 * syy + r02 * r02 * szz;
     const float cw01 = r00 * r10 * sxx + r01 * r11 * syy + r02 * r12 * szz;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void k_eval_project(const float* __restrict__ 
     cyy = cyy + C.blur;
     const float det_bl = fmaxf(cxx * cyy - cxy * cxy, 1e-12f);
     keep = keep && !(det_raw <= 0.0f);
-    const float opacity = 1.0f / (1.0f + rgs_exp(-P[10 * (size_t)npad + g])) * sqrtf(det_raw / det_bl);
+    const float opacity = 1.0f / (1.0f + rgs_expf_libm(-P[10 * (size_t)npad + g])) * sqrtf(det_raw / det_bl);
     keep = keep && !(opacity < 1.0f / 255.0f);
     const float det = cxx * cyy - cxy * cxy;  // the raster loop's determinant
     keep = keep && det >= 1e-10f && __builtin_isfinite(det) && __builtin_isfinite(px) &&
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(256) void k_eval_raster(const float4* __restrict__ 
         const float dx = A.x - fpx, dy = A.y - fpy;
         const float d2 = (B.z * dx * dx - B.y * dx * dy + B.x * dy * dy) * B.w;
         if (!done && d2 < 9.0f) {
-          const float w = rgs_exp(-0.5f * d2) * A.z;
+          const float w = rgs_expf_libm(-0.5f * d2) * A.z;
           if (w > 0.001f) {
             const float a = w * (1.0f - alpha);
             if (a > 0.0f) {

@@ -133,6 +133,45 @@ RGS_HD float rgs_log(float x) {
 // 1 / (1 + e^-x), the logistic function used for opacities.
 RGS_HD float rgs_sigmoid(float x) { return 1.0f / (1.0f + rgs_exp(-x)); }
 
+// expf with the bits of the C library the reference's CPU renderer links
+// (Rust's f32::exp calls it): the table-driven method of glibc's expf
+// (ARM optimized-routines, Szabolcs Nagy 2017).  In double precision,
+// x * 32/ln2 = k + r with k rounded to an integer through the 1.5 * 2^52
+// shift, 2^(k/32) from a 32-entry table of 2^(i/32) (correctly rounded
+// doubles, each stored minus i << 47 so that adding k << 47 also forms the
+// exponent), and e^(r ln2/32) by a degree-3 polynomial; the double result is
+// rounded to float once.  Each step is a correctly rounded IEEE operation,
+// so host and device agree bit for bit; against this image's libm expf it
+// differs on 2 of the 2^32 inputs (x = 0x1.04845ep+5 and -0x1.f8cbb2p+5,
+// one float ulp, both outside |x| < 32), checked exhaustively by the tests.
+// Overflow (x > 0x1.62e42ep6) gives +inf, underflow (x < -0x1.9fe368p6)
+// gives 0, NaN propagates.
+RGS_HD float rgs_expf_libm(float x) {
+  constexpr uint64_t kTab[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull,
+  };
+  if (x != x) return x + x;
+  if (x > 0x1.62e42ep6f) return rgs_u2f(0x7f800000u);
+  if (x < -0x1.9fe368p6f) return 0.0f;
+  const double z = 0x1.71547652b82fep+0 * 32.0 * (double)x;
+  double kd = z + 0x1.8p+52;
+  const uint64_t ki = __builtin_bit_cast(uint64_t, kd);
+  kd -= 0x1.8p+52;
+  const double r = z - kd;
+  const double s = __builtin_bit_cast(double, kTab[ki % 32] + (ki << 47));
+  const double p = __builtin_fma(0x1.c6af84b912394p-5 / 32768.0, r, 0x1.ebfce50fac4f3p-3 / 1024.0);
+  const double y = __builtin_fma(p, r * r, __builtin_fma(0x1.62e42ff0c52d6p-1 / 32.0, r, 1.0));
+  return (float)(y * s);
+}
+
+
 // 1 / sqrt(x) as two correctly rounded operations.
 RGS_HD float rgs_rsqrt(float x) { return 1.0f / rgs_sqrt(x); }
 

@@ -6,8 +6,8 @@
   of the oracle, each gradient group within 1e-3 (rel-L2) of the oracle's
   backward summed over the views.
 * The evaluation render (the CPU renderer's semantics) of the same scene's
-  1280x720 view: colour and depth equal to the oracle's CPU renderer with
-  the library's exp bit for bit, and within 1e-4 of it with libm's expf.
+  1280x720 view: colour and depth equal to the oracle's CPU renderer (with
+  libm's expf, as the reference) bit for bit.
 * Single-view training (1,048,576 Gaussians, 32 ring views at 1280x720, one
   view and one Adam update per step, graph replayed): the first step's
   gradients equal the oracle's, 40 replayed steps lower the L1 loss over the
@@ -110,14 +110,10 @@ def test_config4_evaluation_render(hip_ctx, orc):
     gc, gd = TiledRenderer(cam.width, cam.height).with_raster_cov_blur(0.3).render_camera_splats(hip_ctx, ds, cam)
     gc, gd = gc.cpu().numpy(), gd.cpu().numpy()
     ds.close()
-    oc, od = orc.tiled_renderer(init, cam, cov_blur=0.3, gpu_math=True)
+    # the reference's own arithmetic (libm expf)
+    oc, od = orc.tiled_renderer(init, cam, cov_blur=0.3)
+    say(f"  {int((gc != oc).sum())} colour values and {int((gd != od).sum())} depth pixels differ from the oracle")
     assert np.array_equal(gc, oc) and np.array_equal(gd, od)
-    # the reference's own arithmetic (libm expf): ulps of exp apart
-    lc, ld = orc.tiled_renderer(init, cam, cov_blur=0.3)
-    err = float(np.abs(gc - lc).max())
-    say(f"  bit-identical to the oracle; against libm exp max colour difference {err:.1e}, "
-        f"{int((gd != ld).sum())} depth pixels differ")
-    assert err <= 1e-4
 
 
 def test_config3_graph_replayed_training(hip_ctx, orc):

@@ -2,14 +2,14 @@
 against the oracle:
 
 * rgs_tiled_render (TiledRenderer::render_camera_splats,
-  render/tiled_renderer.rs:174-492) equals the oracle's restatement run with
-  the library's exponential (orc_tiled_renderer, gpu_math=1) BIT FOR BIT,
-  colour and depth, for SH degrees 0-4, a non-black background, ragged
-  image sizes (1x1, one-pixel rows and columns, non-multiples of 16) and
-  edge rows (non-finite parameters, splats behind the camera, one covering
-  every tile, equal depths, vanishing scales).  The oracle with libm's exp
-  (the reference's arithmetic) is pinned against it in
-  tests/test_oracle_evaluation.py.
+  render/tiled_renderer.rs:174-492) equals the oracle's restatement with
+  libm's expf, the reference's own arithmetic (orc_tiled_renderer,
+  gpu_math=0), BIT FOR BIT, colour and depth, for SH degrees 0-4, a
+  non-black background, ragged image sizes (1x1, one-pixel rows and
+  columns, non-multiples of 16) and edge rows (non-finite parameters, splats
+  behind the camera, one covering every tile, equal depths, vanishing
+  scales).  The device's exp (rgs_expf_libm) is checked against libm's
+  expf on every float in tests/test_oracle_math.py.
 * rgs_render_splats (GaussianRenderer::render_splats, renderer.rs:72-98):
   u8 colour and zero-mapped depth, bit for bit.
 * rgs_image_metrics: PSNR and sharpness against the oracle's float64
@@ -59,7 +59,7 @@ def test_tiled_render_bit_exact(hip_ctx, orc, deg):
     for i, cam in enumerate(cams):
         say(f"tiled render deg={deg} view={i} {cam.width}x{cam.height}")
         c, d = _tiled(hip_ctx, sc, cam, bg)
-        rc, rd = orc.tiled_renderer(sc, cam, background=bg, cov_blur=0.3, gpu_math=True)
+        rc, rd = orc.tiled_renderer(sc, cam, background=bg, cov_blur=0.3, gpu_math=False)
         assert (rd < F(3.4028235e38)).mean() > 0.05  # the view is not empty
         _assert_same(c, d, rc, rd, f"deg {deg} view {i}")
 
@@ -70,7 +70,7 @@ def test_tiled_render_ragged_sizes(hip_ctx, orc, wh):
     cam = scene.ring_cameras([wh], radius=3.0, seed=421)[0]
     say(f"tiled render ragged {wh}")
     c, d = _tiled(hip_ctx, sc, cam, (0.3, 0.0, 1.0), blur=0.0)
-    rc, rd = orc.tiled_renderer(sc, cam, background=(0.3, 0.0, 1.0), cov_blur=0.0, gpu_math=True)
+    rc, rd = orc.tiled_renderer(sc, cam, background=(0.3, 0.0, 1.0), cov_blur=0.0, gpu_math=False)
     _assert_same(c, d, rc, rd, f"{wh}")
 
 
@@ -105,7 +105,7 @@ def test_tiled_render_edge_rows(hip_ctx, orc):
     for i, cam in enumerate(cams):
         say(f"tiled render edge rows view={i}")
         c, d = _tiled(hip_ctx, sc, cam, (0.2, 0.2, 0.2))
-        rc, rd = orc.tiled_renderer(sc, cam, background=(0.2, 0.2, 0.2), cov_blur=0.3, gpu_math=True)
+        rc, rd = orc.tiled_renderer(sc, cam, background=(0.2, 0.2, 0.2), cov_blur=0.3, gpu_math=False)
         same_nan = np.isnan(c) == np.isnan(rc)
         assert same_nan.all()
         _assert_same(np.nan_to_num(c, nan=-1.0), d, np.nan_to_num(rc, nan=-1.0), rd, f"edge view {i}")
@@ -124,7 +124,7 @@ def test_tiled_render_large_scene(hip_ctx, orc):
     cam = scene.ring_cameras([(640, 480)], seed=451)[0]
     say("tiled render 65536 splats 640x480")
     c, d = _tiled(hip_ctx, sc, cam)
-    rc, rd = orc.tiled_renderer(sc, cam, cov_blur=0.3, gpu_math=True)
+    rc, rd = orc.tiled_renderer(sc, cam, cov_blur=0.3, gpu_math=False)
     _assert_same(c, d, rc, rd, "65536 splats")
 
 
@@ -138,7 +138,7 @@ def test_gaussian_renderer_u8(hip_ctx, orc):
     out = GaussianRenderer(cam.width, cam.height).with_background(0.0, 0.5, 1.0).with_raster_cov_blur(0.2) \
         .render_splats(hip_ctx, ds, cam)
     ds.close()
-    rc8, rd = orc.gaussian_renderer(sc, cam, background=(0.0, 0.5, 1.0), cov_blur=0.2, gpu_math=True)
+    rc8, rd = orc.gaussian_renderer(sc, cam, background=(0.0, 0.5, 1.0), cov_blur=0.2, gpu_math=False)
     c8, d = out.color.cpu().numpy(), out.depth.cpu().numpy()
     say(f"gaussian renderer u8: {int((c8 != rc8).sum())} bytes differ")
     assert np.array_equal(c8, rc8) and np.array_equal(d.view(np.uint32), rd.view(np.uint32))
@@ -195,7 +195,7 @@ def test_evaluate_frame_and_splats(hip_ctx, orc):
     ds = DeviceSplats(hip_ctx, sc)
     say("evaluate frame")
     m, c8 = evaluate_frame(hip_ctx, ds, cams[0], targets[0], 0.3, keep_color=True)
-    rc8, _ = orc.gaussian_renderer(sc, cams[0], cov_blur=0.3, gpu_math=True)
+    rc8, _ = orc.gaussian_renderer(sc, cams[0], cov_blur=0.3, gpu_math=False)
     assert np.array_equal(c8.cpu().numpy(), rc8)
     rendered = SplatEvaluationRenderer(320, 240, 0.3).render(hip_ctx, ds, cams[0])
     exact = OE.rendered_from_u8(rc8)

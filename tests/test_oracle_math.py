@@ -30,6 +30,16 @@ def test_exp_accuracy(orc):
     assert orc.math_exp(np.array([0.0], np.float32))[0] == 1.0
 
 
+def test_expf_libm_equals_the_c_library_on_every_float(orc):
+    """The evaluation render's exp (rgs_expf_libm, shared by the device and the
+    oracle) against this C library's expf, which Rust's f32::exp calls, over
+    all 2^32 inputs: equal bits except on two inputs (one float ulp each),
+    both outside |x| < 32."""
+    n, xs = orc.expf_libm_check()
+    assert n <= 2, (n, xs)
+    assert all(abs(float(x)) > 32.0 for x in xs), xs
+
+
 def test_exp_special(orc):
     x = np.array([-88.0, -1000.0, 89.0, 1000.0, np.nan], np.float32)
     y = orc.math_exp(x)
